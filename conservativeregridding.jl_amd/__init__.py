@@ -1,0 +1,28 @@
+"""MI355X-native conservative regridding (the two hot stages of
+ConservativeRegridding.jl): the device ``Regridder`` build and ``regrid!``.
+
+Loaded from its folder by path (the folder name is not a Python identifier):
+``__graft_entry__.load_package()`` returns this module.
+"""
+from ._lib import CRError, LIB_PATH, lib as _lib_handle
+from .errors import ArgumentError, DimensionMismatch, MethodError
+from .grids import (Grid, NESTED, RING, as_grid, cubed_sphere_points, healpix_cells,
+                    healpix_nest2ring, healpix_ring2nest, lonlat_points, points_to_cells)
+from .regridder import (EARTH_RADIUS, Regridder, areas, normalize_, regrid, regrid_, size,
+                        transpose)
+
+__all__ = [
+    "Regridder", "regrid_", "regrid", "transpose", "normalize_", "areas", "size",
+    "Grid", "NESTED", "RING", "as_grid", "lonlat_points", "cubed_sphere_points", "healpix_cells",
+    "healpix_nest2ring", "healpix_ring2nest", "points_to_cells",
+    "ArgumentError", "DimensionMismatch", "MethodError", "CRError", "EARTH_RADIUS", "LIB_PATH",
+    "device_count", "version",
+]
+
+
+def device_count() -> int:
+    return int(_lib_handle.cr_device_count())
+
+
+def version() -> str:
+    return _lib_handle.cr_version().decode()

@@ -1,0 +1,135 @@
+"""ctypes binding of libcrhip.so (the gfx950 C ABI declared in include/crhip.h).
+
+The library is required: importing this module raises if libcrhip.so has not
+been built (``python -c "import __graft_entry__ as g; g.build()"``).  There is
+no CPU fallback for any compute entry point.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcrhip.so")
+
+c_i32, c_i64, c_dbl, c_vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+
+CR_OK = 0
+CR_INVALID_ARGUMENT = 1
+CR_DIMENSION_MISMATCH = 2
+CR_UNSUPPORTED = 3
+CR_OUT_OF_MEMORY = 4
+CR_HIP_ERROR = 5
+CR_DEGENERATE_GEOMETRY = 6
+CR_INTERNAL = 7
+CR_NO_DEVICE = 8
+
+CR_F64, CR_F32 = 0, 1
+CR_HEALPIX_NESTED, CR_HEALPIX_RING = 0, 1
+CR_CLIP_AUTO, CR_CLIP_WITH_DST, CR_CLIP_WITH_SRC = 0, 1, 2
+
+
+class BuildOpts(ctypes.Structure):
+    _fields_ = [("threshold_rel", c_dbl), ("radius", c_dbl), ("normalize", c_i32),
+                ("clip_side", c_i32), ("reserved0", c_i32), ("reserved1", c_i32)]
+
+
+class BuildStats(ctypes.Structure):
+    _fields_ = [("n_dst", c_i64), ("n_src", c_i64), ("nnz", c_i64), ("n_candidates", c_i64),
+                ("ms_total", c_dbl), ("ms_bounds", c_dbl), ("ms_tree", c_dbl), ("ms_search", c_dbl),
+                ("ms_clip", c_dbl), ("ms_assemble", c_dbl), ("clip_with_dst", c_i32),
+                ("flipped_cells", c_i32)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+# name -> (restype, argtypes); exactly the entry points of include/crhip.h
+SIGNATURES = {
+    "cr_last_error": (ctypes.c_char_p, []),
+    "cr_version": (ctypes.c_char_p, []),
+    "cr_device_count": (c_i32, []),
+    "cr_gen_lonlat_points": (c_i32, [_dp, c_i64, c_i64, c_dbl, c_dbl, c_dbl, c_dbl]),
+    "cr_gen_cubed_sphere_points": (c_i32, [_dp, c_i64]),
+    "cr_gen_healpix_cells": (c_i32, [_dp, c_i64, c_i32, c_i32, c_i64, c_i64]),
+    "cr_healpix_nest2ring": (c_i32, [c_i64, c_i64, _i64p]),
+    "cr_healpix_ring2nest": (c_i32, [c_i64, c_i64, _i64p]),
+    "cr_points_to_cells": (c_i32, [_dp, c_i64, c_i64, c_i64, _dp]),
+    "cr_grid_from_cells": (c_i32, [ctypes.POINTER(c_vp), _dp, c_i64, c_i32, c_i32, c_vp]),
+    "cr_grid_from_points": (c_i32, [ctypes.POINTER(c_vp), _dp, c_i64, c_i64, c_i64, c_vp]),
+    "cr_grid_healpix": (c_i32, [ctypes.POINTER(c_vp), c_i64, c_i32, c_i32, c_vp]),
+    "cr_grid_info": (c_i32, [c_vp, _i64p, _i32p, _i32p]),
+    "cr_grid_copy_cells": (c_i32, [c_vp, _dp]),
+    "cr_grid_areas": (c_i32, [c_vp, _dp]),
+    "cr_grid_destroy": (None, [c_vp]),
+    "cr_build_opts_default": (None, [ctypes.POINTER(BuildOpts)]),
+    "cr_regridder_build": (c_i32, [ctypes.POINTER(c_vp), c_vp, c_vp, ctypes.POINTER(BuildOpts), c_vp]),
+    "cr_regridder_from_csr": (c_i32, [ctypes.POINTER(c_vp), c_i64, c_i64, c_i64, _i64p, _i32p, _dp,
+                                      _dp, _dp, c_vp]),
+    "cr_regridder_info": (c_i32, [c_vp, _i64p, _i64p, _i64p]),
+    "cr_regridder_stats": (c_i32, [c_vp, ctypes.POINTER(BuildStats)]),
+    "cr_regridder_copy_csr": (c_i32, [c_vp, c_i32, _i64p, _i32p, _dp]),
+    "cr_regridder_copy_areas": (c_i32, [c_vp, _dp, _dp]),
+    "cr_regridder_normalize": (c_i32, [c_vp, c_vp]),
+    "cr_regridder_destroy": (None, [c_vp]),
+    "cr_regrid": (c_i32, [c_vp, c_i32, c_i32, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, c_i32,
+                          c_vp]),
+}
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built: the HIP library is required (no CPU fallback). "
+            "Run __graft_entry__.build() first.")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
+lib = _load()
+
+
+class CRError(RuntimeError):
+    """A failing library call (status + message from cr_last_error)."""
+
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"[status {status}] {msg}")
+        self.status = status
+
+
+def last_error() -> str:
+    m = lib.cr_last_error()
+    return m.decode() if m else ""
+
+
+def check(status: int, what: str = ""):
+    if status == CR_OK:
+        return
+    from .errors import ArgumentError, DimensionMismatch, MethodError
+    msg = f"{what}: {last_error()}" if what else last_error()
+    if status == CR_INVALID_ARGUMENT:
+        raise ArgumentError(msg)
+    if status == CR_DIMENSION_MISMATCH:
+        raise DimensionMismatch(msg)
+    if status == CR_UNSUPPORTED:
+        raise MethodError(msg)
+    raise CRError(status, msg)
+
+
+def dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def i64ptr(a):
+    return a.ctypes.data_as(_i64p)
+
+
+def i32ptr(a):
+    return a.ctypes.data_as(_i32p)

@@ -1,0 +1,648 @@
+// build.hip -- Regridder(dst, src) on the device.
+//
+// Reference pipeline (src/regridder/regridder.jl:125-163,
+// src/regridder/intersection_areas.jl:67-157) and what replaces each step:
+//   treeify + dual-tree DFS over spherical caps (MultithreadedDualDepthFirstSearch.jl)
+//     -> k_cell_bounds: FP64 area + bounding cap of every cell
+//        Morton sort of source caps + implicit 8-ary tree of caps (k_build_level)
+//        k_traverse<false/true>: one thread per destination cell walks the tree
+//        stacklessly; count pass + scan + fill pass emit candidates grouped by
+//        destination row in a deterministic order
+//   compute_intersection_areas with ConvexConvexSutherlandHodgman + GO.area
+//     -> k_clip<4>: one thread per candidate, streaming FP64 clip + area
+//   keep iff area > 0 (here > threshold_rel * min(a_src, a_dst)) and
+//   sparse(dst, src, areas)
+//     -> k_keep_scatter: wave64 ballot compaction of kept pairs (row-grouped COO)
+//        two stable radix sorts: by column (gives CSR(A^T)) then by row (gives
+//        CSR(A) with sorted columns)
+//   areas(manifold, tree)  -> the k_cell_bounds areas, times radius^2
+//   normalize!             -> k_scale by 1 / max(A)
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <vector>
+
+#include "regridder.hpp"
+#include "sphgeom.hpp"
+
+namespace cr {
+
+constexpr int kTreeFanout = 8;
+constexpr int kMaxLevels = 24;
+constexpr float kCapAll = 4.0f;  // radius that overlaps everything (chord <= 2)
+
+struct Levels {
+    int64_t off[kMaxLevels];
+    int64_t size[kMaxLevels];
+    int top;
+};
+
+// Area (unit sphere) and bounding cap of each cell.  The cap is a ball
+// (centre, chord radius) around the normalised vertex mean; for cells inside a
+// hemisphere it holds the whole cell.  FP32 copies are padded outward.
+__global__ void k_cell_bounds(const double *__restrict__ cells, int64_t n, int nv,
+                              double *__restrict__ area, float4 *__restrict__ cap) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const double *p = cells + 3 * nv * c;
+    area[c] = ring_area(p, nv);
+    d3 s = mk3(0, 0, 0);
+    for (int k = 0; k < nv; ++k) s = s + load3(p + 3 * k);
+    double L = sqrt(dot(s, s));
+    if (L < 1e-12) {
+        cap[c] = make_float4(0.f, 0.f, 1.f, kCapAll);
+        return;
+    }
+    d3 m = (1.0 / L) * s;
+    double r = 0.0;
+    for (int k = 0; k < nv; ++k) {
+        d3 d = load3(p + 3 * k) - m;
+        r = fmax(r, sqrt(dot(d, d)));
+    }
+    float rf = r > 1.4 ? kCapAll : (float)(r * (1.0 + 1e-6) + 2e-6);
+    cap[c] = make_float4((float)m.x, (float)m.y, (float)m.z, rf);
+}
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ void k_morton(const float4 *__restrict__ cap, int64_t n, uint32_t *__restrict__ key,
+                         int32_t *__restrict__ idx) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    float4 q = cap[c];
+    auto quant = [](float x) {
+        int v = (int)((x + 1.0f) * 512.0f);
+        return (uint32_t)min(1023, max(0, v));
+    };
+    key[c] = spread3(quant(q.x)) | (spread3(quant(q.y)) << 1) | (spread3(quant(q.z)) << 2);
+    idx[c] = (int32_t)c;
+}
+
+__global__ void k_gather_caps(const float4 *__restrict__ cap, const int32_t *__restrict__ perm,
+                              int64_t n, float4 *__restrict__ leaf) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) leaf[i] = cap[perm[i]];
+}
+
+// Parent cap = ball around the normalised mean of the child centres that holds
+// every child ball (triangle inequality on chords).
+__global__ void k_build_level(const float4 *__restrict__ child, int64_t nchild,
+                              float4 *__restrict__ parent, int64_t nparent) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nparent) return;
+    int64_t c0 = p * kTreeFanout, c1 = min(nchild, c0 + kTreeFanout);
+    d3 s = mk3(0, 0, 0);
+    bool all = false;
+    for (int64_t c = c0; c < c1; ++c) {
+        float4 q = child[c];
+        s = s + mk3(q.x, q.y, q.z);
+        all |= q.w >= 2.0f;
+    }
+    double L = sqrt(dot(s, s));
+    if (all || L < 1e-9) {
+        parent[p] = make_float4(0.f, 0.f, 1.f, kCapAll);
+        return;
+    }
+    d3 m = (1.0 / L) * s;
+    double r = 0.0;
+    for (int64_t c = c0; c < c1; ++c) {
+        float4 q = child[c];
+        d3 d = mk3(q.x, q.y, q.z) - m;
+        r = fmax(r, sqrt(dot(d, d)) + (double)q.w);
+    }
+    float rf = r >= 2.0 ? kCapAll : (float)(r * (1.0 + 1e-6) + 2e-6);
+    parent[p] = make_float4((float)m.x, (float)m.y, (float)m.z, rf);
+}
+
+__device__ __forceinline__ bool caps_overlap(const float4 &a, const float4 &b) {
+    float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    float rr = a.w + b.w;
+    return dx * dx + dy * dy + dz * dz <= rr * rr;
+}
+
+// One thread per destination cell; stackless depth-first walk of the implicit
+// kTreeFanout-ary tree (level 0 = Morton-sorted source cells).  FILL=false
+// counts candidates, FILL=true writes them at the scanned offsets.
+template <bool FILL>
+__global__ void __launch_bounds__(256) k_traverse(const float4 *__restrict__ qcap, int64_t nq,
+                                                  const float4 *__restrict__ nodes, Levels lv,
+                                                  const int32_t *__restrict__ perm,
+                                                  int64_t *__restrict__ count_or_offset,
+                                                  int32_t *__restrict__ cand_src,
+                                                  int32_t *__restrict__ cand_dst) {
+    int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= nq) return;
+    float4 q = qcap[d];
+    int64_t pos = FILL ? count_or_offset[d] : 0;
+    const int64_t end = FILL ? count_or_offset[d + 1] : 0;  // never write past the row
+    int64_t cnt = 0;
+    int L = lv.top;
+    int64_t i = 0;
+    for (;;) {
+        bool hit = caps_overlap(q, nodes[lv.off[L] + i]);
+        if (hit) {
+            if (L == 0) {
+                if (FILL) {
+                    if (pos < end) {
+                        cand_src[pos] = perm[i];
+                        cand_dst[pos] = (int32_t)d;
+                        ++pos;
+                    }
+                } else {
+                    ++cnt;
+                }
+            } else {
+                --L;
+                i *= kTreeFanout;
+                continue;
+            }
+        }
+        // next sibling, or climb until one exists
+        for (;;) {
+            if (L == lv.top) goto done;
+            int64_t nxt = i + 1;
+            if ((nxt % kTreeFanout) != 0 && nxt < lv.size[L]) {
+                i = nxt;
+                break;
+            }
+            i /= kTreeFanout;
+            ++L;
+        }
+    }
+done:
+    if (!FILL) count_or_offset[d] = cnt;
+}
+
+// One thread per candidate: FP64 area of (subject ∩ clip) via the streaming
+// clip pipeline; stores the area if kept, else 0.
+template <int NC>
+__global__ void __launch_bounds__(256) k_clip(const int32_t *__restrict__ cand_dst,
+                                              const int32_t *__restrict__ cand_src, int64_t ncand,
+                                              const double *__restrict__ dst_cells, int dst_nv,
+                                              const double *__restrict__ src_cells, int src_nv,
+                                              int clip_is_dst, const double *__restrict__ a_dst,
+                                              const double *__restrict__ a_src, double thr,
+                                              double *__restrict__ val) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncand) return;
+    int64_t d = cand_dst[c], s = cand_src[c];
+    const double *dp = dst_cells + 3 * (int64_t)dst_nv * d;
+    const double *sp = src_cells + 3 * (int64_t)src_nv * s;
+    double a = clip_is_dst ? clip_area_stream<NC>(dp, dst_nv, sp, src_nv)
+                           : clip_area_stream<NC>(sp, src_nv, dp, dst_nv);
+    double amin = fmin(a_src[s], a_dst[d]);
+    val[c] = a > thr * amin ? a : 0.0;
+}
+
+// Generic clip for convex clip polygons with more than 4 vertices: classic
+// buffered Sutherland-Hodgman in private memory.
+constexpr int kGenericMaxV = 96;
+
+__device__ double clip_area_buffered(const double *clip, int nc, const double *subj, int ns) {
+    d3 bufA[kGenericMaxV], bufB[kGenericMaxV];
+    int m = min(ns, kGenericMaxV);
+    for (int i = 0; i < m; ++i) bufA[i] = load3(subj + 3 * i);
+    d3 *cur = bufA, *nxt = bufB;
+    for (int k = 0; k < nc && m > 0; ++k) {
+        d3 a = load3(clip + 3 * k), b = load3(clip + 3 * ((k + 1) % nc));
+        d3 n = cross(a, b - a);
+        if (dot(n, n) < kDegenerateEdge2) continue;
+        int w = 0;
+        d3 p = cur[m - 1];
+        double sp = dot(n, p - a);
+        for (int i = 0; i < m && w < kGenericMaxV - 2; ++i) {
+            d3 q = cur[i];
+            double sq = dot(n, q - a);
+            if (sq >= 0.0) {
+                if (sp < 0.0) nxt[w++] = crossing(p, sp, q, sq);
+                nxt[w++] = q;
+            } else if (sp >= 0.0) {
+                nxt[w++] = crossing(p, sp, q, sq);
+            }
+            p = q;
+            sp = sq;
+        }
+        m = w;
+        d3 *t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    AreaAcc acc;
+    acc.init();
+    for (int i = 0; i < m; ++i) acc.push(cur[i]);
+    return acc.area();
+}
+
+__global__ void k_clip_generic(const int32_t *__restrict__ cand_dst,
+                               const int32_t *__restrict__ cand_src, int64_t ncand,
+                               const double *__restrict__ dst_cells, int dst_nv,
+                               const double *__restrict__ src_cells, int src_nv, int clip_is_dst,
+                               const double *__restrict__ a_dst, const double *__restrict__ a_src,
+                               double thr, double *__restrict__ val) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncand) return;
+    int64_t d = cand_dst[c], s = cand_src[c];
+    const double *dp = dst_cells + 3 * (int64_t)dst_nv * d;
+    const double *sp = src_cells + 3 * (int64_t)src_nv * s;
+    double a = clip_is_dst ? clip_area_buffered(dp, dst_nv, sp, src_nv)
+                           : clip_area_buffered(sp, src_nv, dp, dst_nv);
+    double amin = fmin(a_src[s], a_dst[d]);
+    val[c] = a > thr * amin ? a : 0.0;
+}
+
+// ---- wave64 ballot compaction of kept candidates --------------------------
+constexpr int kCompactBlock = 256;
+
+__global__ void __launch_bounds__(kCompactBlock) k_keep_count(const double *__restrict__ val,
+                                                              int64_t n,
+                                                              int64_t *__restrict__ block_cnt) {
+    __shared__ int wave_cnt[kCompactBlock / 64];
+    int64_t c = (int64_t)blockIdx.x * kCompactBlock + threadIdx.x;
+    bool keep = c < n && val[c] > 0.0;
+    uint64_t m = __ballot(keep);
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) wave_cnt[wave] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < kCompactBlock / 64; ++w) t += wave_cnt[w];
+        block_cnt[blockIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(kCompactBlock) k_keep_scatter(
+    const double *__restrict__ val, const int32_t *__restrict__ cand_dst,
+    const int32_t *__restrict__ cand_src, int64_t n, const int64_t *__restrict__ block_off,
+    int32_t *__restrict__ out_row, int32_t *__restrict__ out_col, double *__restrict__ out_val,
+    unsigned long long *__restrict__ row_cnt) {
+    __shared__ int wave_base[kCompactBlock / 64];
+    int64_t c = (int64_t)blockIdx.x * kCompactBlock + threadIdx.x;
+    double v = c < n ? val[c] : 0.0;
+    bool keep = v > 0.0;
+    uint64_t m = __ballot(keep);
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) wave_base[wave] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < kCompactBlock / 64; ++w) {
+            int x = wave_base[w];
+            wave_base[w] = t;
+            t += x;
+        }
+    }
+    __syncthreads();
+    if (keep) {
+        uint64_t below = lane ? (m & ((~0ull) >> (64 - lane))) : 0ull;
+        int64_t o = block_off[blockIdx.x] + wave_base[wave] + __popcll(below);
+        int32_t r = cand_dst[c];
+        out_row[o] = r;
+        out_col[o] = cand_src[c];
+        out_val[o] = v;
+        atomicAdd(&row_cnt[r], 1ull);
+    }
+}
+
+__global__ void k_histogram(const int32_t *__restrict__ key, int64_t n,
+                            unsigned long long *__restrict__ cnt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[key[i]], 1ull);
+}
+
+__global__ void k_iota(int32_t *__restrict__ x, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = (int32_t)i;
+}
+
+template <class T>
+__global__ void k_gather(const T *__restrict__ src, const int32_t *__restrict__ idx, int64_t n,
+                         T *__restrict__ dst) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
+__global__ void k_scale(double *__restrict__ x, int64_t n, double mul, double div) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = (x[i] * mul) / div;
+}
+
+struct Timer {
+    hipStream_t s;
+    std::vector<hipEvent_t> ev;
+    explicit Timer(hipStream_t st) : s(st) {}
+    void mark() {
+        hipEvent_t e;
+        CR_HIP(hipEventCreate(&e));
+        CR_HIP(hipEventRecord(e, s));
+        ev.push_back(e);
+    }
+    double ms(int a, int b) const {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, ev[a], ev[b]) != hipSuccess) return 0.0;
+        return t;
+    }
+    ~Timer() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+// hipcub temporary storage, grown on demand.
+struct Temp {
+    DevBuf<unsigned char> buf;
+    void *get(size_t bytes) {
+        if (buf.n < bytes) buf.alloc(std::max(bytes, (size_t)1));
+        return buf.p;
+    }
+};
+
+template <class In, class Out>
+void exclusive_sum(Temp &T, In in, Out out, int64_t n, hipStream_t s) {
+    size_t bytes = 0;
+    CR_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, s));
+    void *p = T.get(bytes);
+    CR_HIP(hipcub::DeviceScan::ExclusiveSum(p, bytes, in, out, (int)n, s));
+}
+
+template <class K, class V>
+void sort_pairs(Temp &T, const K *kin, K *kout, const V *vin, V *vout, int64_t n, int end_bit,
+                hipStream_t s) {
+    size_t bytes = 0;
+    CR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, (int)n, 0,
+                                              end_bit, s));
+    void *p = T.get(bytes);
+    CR_HIP(hipcub::DeviceRadixSort::SortPairs(p, bytes, kin, kout, vin, vout, (int)n, 0, end_bit, s));
+}
+
+int bits_for(int64_t n) {
+    int b = 1;
+    while ((int64_t(1) << b) < n) ++b;
+    return b;
+}
+
+void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
+                     const cr_build_opts &o, hipStream_t s) {
+    const int64_t nd = dst->ncells, ns = src->ncells;
+    const int B = 256;
+    Timer tm(s);
+    Temp T;
+    tm.mark();  // 0
+
+    // clip side: a convex side clips; prefer the destination (GO clips p1=src by p2=dst)
+    int clip_is_dst;
+    if (o.clip_side == CR_CLIP_WITH_DST) clip_is_dst = 1;
+    else if (o.clip_side == CR_CLIP_WITH_SRC) clip_is_dst = 0;
+    else {
+        CR_REQUIRE(dst->convex || src->convex, CR_UNSUPPORTED,
+                   "neither grid is declared convex: the Sutherland-Hodgman clip needs a convex side");
+        clip_is_dst = dst->convex ? 1 : 0;
+    }
+    const int clip_nv = clip_is_dst ? dst->nv : src->nv;
+
+    // 1. areas + caps
+    DevBuf<double> a_dst(nd), a_src(ns);
+    DevBuf<float4> cap_dst(nd), cap_src(ns);
+    hipLaunchKernelGGL(k_cell_bounds, dim3(grid_for(nd, B)), dim3(B), 0, s, dst->cells.p, nd, dst->nv,
+                       a_dst.p, cap_dst.p);
+    hipLaunchKernelGGL(k_cell_bounds, dim3(grid_for(ns, B)), dim3(B), 0, s, src->cells.p, ns, src->nv,
+                       a_src.p, cap_src.p);
+    CR_HIP(hipGetLastError());
+    tm.mark();  // 1
+
+    // 2. tree over source caps
+    Levels lv{};
+    {
+        int64_t n = ns, off = 0;
+        int L = 0;
+        for (;;) {
+            CR_REQUIRE(L < kMaxLevels, CR_INTERNAL, "tree too deep");
+            lv.off[L] = off;
+            lv.size[L] = n;
+            off += n;
+            if (n == 1) break;
+            n = (n + kTreeFanout - 1) / kTreeFanout;
+            ++L;
+        }
+        lv.top = L;
+    }
+    int64_t nnodes = lv.off[lv.top] + 1;
+    DevBuf<float4> nodes(nnodes);
+    DevBuf<int32_t> perm(ns);
+    {
+        DevBuf<uint32_t> key(ns), key_sorted(ns);
+        DevBuf<int32_t> idx(ns);
+        hipLaunchKernelGGL(k_morton, dim3(grid_for(ns, B)), dim3(B), 0, s, cap_src.p, ns, key.p, idx.p);
+        sort_pairs(T, key.p, key_sorted.p, idx.p, perm.p, ns, 30, s);
+        hipLaunchKernelGGL(k_gather_caps, dim3(grid_for(ns, B)), dim3(B), 0, s, cap_src.p, perm.p, ns,
+                           nodes.p);
+        for (int L = 1; L <= lv.top; ++L)
+            hipLaunchKernelGGL(k_build_level, dim3(grid_for(lv.size[L], B)), dim3(B), 0, s,
+                               nodes.p + lv.off[L - 1], lv.size[L - 1], nodes.p + lv.off[L], lv.size[L]);
+        CR_HIP(hipGetLastError());
+    }
+    tm.mark();  // 2
+
+    // 3. candidate search: count, scan, fill
+    DevBuf<int64_t> cand_off(nd + 1);
+    {
+        DevBuf<int64_t> cand_cnt(nd + 1);
+        CR_HIP(hipMemsetAsync(cand_cnt.p + nd, 0, sizeof(int64_t), s));
+        hipLaunchKernelGGL(k_traverse<false>, dim3(grid_for(nd, B)), dim3(B), 0, s, cap_dst.p, nd,
+                           nodes.p, lv, perm.p, cand_cnt.p, (int32_t *)nullptr, (int32_t *)nullptr);
+        CR_HIP(hipGetLastError());
+        exclusive_sum(T, cand_cnt.p, cand_off.p, nd + 1, s);
+    }
+    int64_t ncand = 0;
+    CR_HIP(hipMemcpyAsync(&ncand, cand_off.p + nd, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    CR_HIP(hipStreamSynchronize(s));
+    CR_REQUIRE(ncand < (int64_t(1) << 31), CR_UNSUPPORTED, "more than 2^31 candidate pairs");
+    DevBuf<int32_t> cand_src(ncand), cand_dst(ncand);
+    if (ncand) {  // zero-fill: any slot the fill pass skipped is the valid pair (0, 0)
+        CR_HIP(hipMemsetAsync(cand_src.p, 0, cand_src.bytes(), s));
+        CR_HIP(hipMemsetAsync(cand_dst.p, 0, cand_dst.bytes(), s));
+    }
+    if (ncand)
+        hipLaunchKernelGGL(k_traverse<true>, dim3(grid_for(nd, B)), dim3(B), 0, s, cap_dst.p, nd,
+                           nodes.p, lv, perm.p, cand_off.p, cand_src.p, cand_dst.p);
+    CR_HIP(hipGetLastError());
+    tm.mark();  // 3
+
+    // 4. clip + area per candidate
+    DevBuf<double> cval(ncand);
+    if (ncand) {
+        if (clip_nv <= 4)
+            hipLaunchKernelGGL(k_clip<4>, dim3(grid_for(ncand, B)), dim3(B), 0, s, cand_dst.p,
+                               cand_src.p, ncand, dst->cells.p, dst->nv, src->cells.p, src->nv,
+                               clip_is_dst, a_dst.p, a_src.p, o.threshold_rel, cval.p);
+        else
+            hipLaunchKernelGGL(k_clip_generic, dim3(grid_for(ncand, 64)), dim3(64), 0, s, cand_dst.p,
+                               cand_src.p, ncand, dst->cells.p, dst->nv, src->cells.p, src->nv,
+                               clip_is_dst, a_dst.p, a_src.p, o.threshold_rel, cval.p);
+        CR_HIP(hipGetLastError());
+    }
+    tm.mark();  // 4
+
+    // 5. compaction (row-grouped COO) and the two CSRs
+    int64_t nblk = (ncand + kCompactBlock - 1) / kCompactBlock;
+    DevBuf<int64_t> blk(nblk + 1);
+    {
+        DevBuf<int64_t> blk_cnt(nblk + 1);
+        CR_HIP(hipMemsetAsync(blk_cnt.p, 0, blk_cnt.bytes(), s));
+        if (ncand)
+            hipLaunchKernelGGL(k_keep_count, dim3((unsigned)nblk), dim3(kCompactBlock), 0, s, cval.p,
+                               ncand, blk_cnt.p);
+        exclusive_sum(T, blk_cnt.p, blk.p, nblk + 1, s);
+    }
+    int64_t nnz = 0;
+    CR_HIP(hipMemcpyAsync(&nnz, blk.p + nblk, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    CR_HIP(hipStreamSynchronize(s));
+
+    Csr &F = R->fwd;
+    Csr &Bk = R->bwd;
+    F.nrows = nd; F.ncols = ns; F.nnz = nnz;
+    Bk.nrows = ns; Bk.ncols = nd; Bk.nnz = nnz;
+    F.rowptr.alloc(nd + 1);
+    Bk.rowptr.alloc(ns + 1);
+    F.col.alloc(nnz); F.val.alloc(nnz);
+    Bk.col.alloc(nnz); Bk.val.alloc(nnz);
+    {
+        DevBuf<int32_t> coo_row(nnz), coo_col(nnz);
+        DevBuf<double> coo_val(nnz);
+        DevBuf<unsigned long long> rcnt(nd + 1), ccnt(ns + 1);
+        CR_HIP(hipMemsetAsync(rcnt.p, 0, rcnt.bytes(), s));
+        CR_HIP(hipMemsetAsync(ccnt.p, 0, ccnt.bytes(), s));
+        if (ncand)
+            hipLaunchKernelGGL(k_keep_scatter, dim3((unsigned)nblk), dim3(kCompactBlock), 0, s, cval.p,
+                               cand_dst.p, cand_src.p, ncand, blk.p, coo_row.p, coo_col.p, coo_val.p,
+                               rcnt.p);
+        hipLaunchKernelGGL(k_histogram, dim3(grid_for(nnz, B)), dim3(B), 0, s, coo_col.p, nnz, ccnt.p);
+        CR_HIP(hipGetLastError());
+        exclusive_sum(T, (int64_t *)rcnt.p, F.rowptr.p, nd + 1, s);
+        exclusive_sum(T, (int64_t *)ccnt.p, Bk.rowptr.p, ns + 1, s);
+        if (nnz) {
+            // stable sort by column: CSR(A^T), rows of A^T sorted by A's row
+            DevBuf<int32_t> iota(nnz), p1(nnz), tkey(nnz), p2(nnz), fkey(nnz);
+            hipLaunchKernelGGL(k_iota, dim3(grid_for(nnz, B)), dim3(B), 0, s, iota.p, nnz);
+            sort_pairs(T, coo_col.p, tkey.p, iota.p, p1.p, nnz, bits_for(ns), s);
+            hipLaunchKernelGGL(k_gather<int32_t>, dim3(grid_for(nnz, B)), dim3(B), 0, s, coo_row.p, p1.p,
+                               nnz, Bk.col.p);
+            hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(nnz, B)), dim3(B), 0, s, coo_val.p, p1.p,
+                               nnz, Bk.val.p);
+            // stable sort of that by row: CSR(A) with sorted columns
+            sort_pairs(T, Bk.col.p, fkey.p, iota.p, p2.p, nnz, bits_for(nd), s);
+            hipLaunchKernelGGL(k_gather<int32_t>, dim3(grid_for(nnz, B)), dim3(B), 0, s, tkey.p, p2.p,
+                               nnz, F.col.p);
+            hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(nnz, B)), dim3(B), 0, s, Bk.val.p, p2.p,
+                               nnz, F.val.p);
+            CR_HIP(hipGetLastError());
+        }
+    }
+
+    // 6. radius^2 scaling, areas
+    const double r2 = o.radius * o.radius;
+    R->a_dst = std::move(a_dst);
+    R->a_src = std::move(a_src);
+    if (r2 != 1.0) {
+        hipLaunchKernelGGL(k_scale, dim3(grid_for(nnz, B)), dim3(B), 0, s, F.val.p, nnz, r2, 1.0);
+        hipLaunchKernelGGL(k_scale, dim3(grid_for(nnz, B)), dim3(B), 0, s, Bk.val.p, nnz, r2, 1.0);
+        hipLaunchKernelGGL(k_scale, dim3(grid_for(nd, B)), dim3(B), 0, s, R->a_dst.p, nd, r2, 1.0);
+        hipLaunchKernelGGL(k_scale, dim3(grid_for(ns, B)), dim3(B), 0, s, R->a_src.p, ns, r2, 1.0);
+        CR_HIP(hipGetLastError());
+    }
+    finalize_regridder(R, s);
+    tm.mark();  // 5
+    CR_HIP(hipStreamSynchronize(s));
+
+    auto &st = R->stats;
+    st.n_dst = nd;
+    st.n_src = ns;
+    st.nnz = nnz;
+    st.n_candidates = ncand;
+    st.ms_bounds = tm.ms(0, 1);
+    st.ms_tree = tm.ms(1, 2);
+    st.ms_search = tm.ms(2, 3);
+    st.ms_clip = tm.ms(3, 4);
+    st.ms_assemble = tm.ms(4, 5);
+    st.ms_total = tm.ms(0, 5);
+    st.clip_with_dst = clip_is_dst;
+    st.flipped_cells = dst->flipped + src->flipped;
+}
+
+// normalize!(R): every stored value and both area vectors divided by max(A).
+void normalize_regridder(cr_regridder *R, hipStream_t s) {
+    Csr &F = R->fwd;
+    if (F.nnz == 0) return;
+    Temp T;
+    DevBuf<double> mx(1);
+    size_t bytes = 0;
+    CR_HIP(hipcub::DeviceReduce::Max(nullptr, bytes, F.val.p, mx.p, (int)F.nnz, s));
+    void *p = T.get(bytes);
+    CR_HIP(hipcub::DeviceReduce::Max(p, bytes, F.val.p, mx.p, (int)F.nnz, s));
+    double m = 0;
+    CR_HIP(hipMemcpyAsync(&m, mx.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    CR_HIP(hipStreamSynchronize(s));
+    CR_REQUIRE(m > 0.0, CR_DEGENERATE_GEOMETRY, "normalize!: maximum(A) is not positive");
+    const int B = 256;
+    hipLaunchKernelGGL(k_scale, dim3(grid_for(F.nnz, B)), dim3(B), 0, s, F.val.p, F.nnz, 1.0, m);
+    hipLaunchKernelGGL(k_scale, dim3(grid_for(R->bwd.nnz, B)), dim3(B), 0, s, R->bwd.val.p, R->bwd.nnz,
+                       1.0, m);
+    hipLaunchKernelGGL(k_scale, dim3(grid_for(F.nrows, B)), dim3(B), 0, s, R->a_dst.p, F.nrows, 1.0, m);
+    hipLaunchKernelGGL(k_scale, dim3(grid_for(F.ncols, B)), dim3(B), 0, s, R->a_src.p, F.ncols, 1.0, m);
+    CR_HIP(hipGetLastError());
+    finalize_regridder(R, s);
+    CR_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace cr
+
+using namespace cr;
+
+extern "C" {
+
+void cr_build_opts_default(cr_build_opts *o) {
+    if (!o) return;
+    o->threshold_rel = 1e-14;
+    o->radius = 6371008.8;
+    o->normalize = 0;
+    o->clip_side = CR_CLIP_AUTO;
+    o->reserved0 = 0;
+    o->reserved1 = 0;
+}
+
+cr_status cr_regridder_build(cr_regridder **R, const cr_grid *dst, const cr_grid *src,
+                             const cr_build_opts *opts, void *stream) {
+    return guard([&] {
+        CR_REQUIRE(R && dst && src, CR_INVALID_ARGUMENT, "cr_regridder_build: null argument");
+        cr_build_opts o;
+        if (opts) o = *opts; else cr_build_opts_default(&o);
+        CR_REQUIRE(o.threshold_rel >= 0.0, CR_INVALID_ARGUMENT, "threshold_rel must be >= 0");
+        CR_REQUIRE(o.radius > 0.0, CR_INVALID_ARGUMENT, "radius must be > 0");
+        CR_REQUIRE(o.clip_side >= 0 && o.clip_side <= 2, CR_INVALID_ARGUMENT, "bad clip_side");
+        check_device();
+        hipStream_t s = as_stream(stream);
+        auto *r = new cr_regridder();
+        try {
+            build_regridder(r, dst, src, o, s);
+            if (o.normalize) normalize_regridder(r, s);
+        } catch (...) {
+            delete r;
+            throw;
+        }
+        *R = r;
+    });
+}
+
+cr_status cr_regridder_normalize(cr_regridder *R, void *stream) {
+    return guard([&] {
+        CR_REQUIRE(R, CR_INVALID_ARGUMENT, "null regridder");
+        normalize_regridder(R, as_stream(stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,105 @@
+// common.hpp -- shared definitions of the HIP library (host + device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/crhip.h"
+
+#define CR_HD __host__ __device__ __forceinline__
+
+namespace cr {
+
+// ------------------------------------------------------------- errors --
+void set_error(const std::string &msg);
+
+struct Error {
+    cr_status status;
+    std::string msg;
+};
+
+#define CR_HIP(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            throw ::cr::Error{e_ == hipErrorOutOfMemory ? CR_OUT_OF_MEMORY : CR_HIP_ERROR, \
+                              std::string(#expr) + ": " + hipGetErrorString(e_)};        \
+    } while (0)
+
+#define CR_REQUIRE(cond, status, msg)                               \
+    do {                                                            \
+        if (!(cond)) throw ::cr::Error{(status), std::string(msg)}; \
+    } while (0)
+
+// Run a body and turn exceptions into a status + thread-local message.
+template <class F>
+cr_status guard(F &&f) {
+    try {
+        f();
+        return CR_OK;
+    } catch (const Error &e) {
+        set_error(e.msg);
+        return e.status;
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed");
+        return CR_OUT_OF_MEMORY;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return CR_INTERNAL;
+    }
+}
+
+// Device memory owned by a handle.
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) CR_HIP(hipMalloc(&p, count * sizeof(T)));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline void check_device() {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) throw Error{CR_NO_DEVICE, "no HIP device is visible"};
+}
+
+inline unsigned grid_for(int64_t n, int block) {
+    int64_t g = (n + block - 1) / block;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+}  // namespace cr
+
+// ------------------------------------------------------------- handles --
+struct cr_grid {
+    int64_t ncells = 0;
+    int32_t nv = 0;
+    int32_t convex = 0;
+    int32_t flipped = 0;
+    cr::DevBuf<double> cells;  // ncells * nv * 3, xyz interleaved per vertex
+};

@@ -1,0 +1,298 @@
+// grids.hip -- grid generators (host entry points) and device grid creation.
+#include <algorithm>
+#include <thread>
+#include <vector>
+
+#include "gridgen.hpp"
+#include "sphgeom.hpp"
+
+namespace cr {
+
+// Cells of point blocks: cell (b,i,j) = P[i,j], P[i+1,j], P[i+1,j+1], P[i,j+1]
+// (CellBasedGrid getcell, src/trees/grids.jl:71-84), index b*ni*nj + i + j*ni
+// (linear_to_cartesian_idx, src/trees/interfaces.jl:236-243).
+__global__ void k_points_to_cells(const double *__restrict__ pts, int64_t nblocks, int64_t ni,
+                                  int64_t nj, double *__restrict__ cells) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t per = ni * nj;
+    if (c >= nblocks * per) return;
+    int64_t b = c / per, r = c - b * per;
+    int64_t j = r / ni, i = r - j * ni;
+    int64_t W = ni + 1;
+    const double *P = pts + 3 * b * W * (nj + 1);
+    int64_t idx[4] = {i + j * W, i + 1 + j * W, i + 1 + (j + 1) * W, i + (j + 1) * W};
+    for (int k = 0; k < 4; ++k)
+        for (int d = 0; d < 3; ++d) cells[(c * 4 + k) * 3 + d] = P[idx[k] * 3 + d];
+}
+
+__global__ void k_healpix_cells(int64_t nside, int order, int step, int64_t first, int64_t count,
+                                double *__restrict__ cells) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int nv = 4 * step;
+    if (t >= count * nv) return;
+    int64_t c = t / nv;
+    int k = (int)(t - c * nv);
+    int f;
+    int64_t ix, iy;
+    hpx_cell_fxy(nside, order, first + c, &f, &ix, &iy);
+    hpx_pixel_vertex(f, ix, iy, nside, step, k, cells + 3 * t);
+}
+
+// Re-orient clockwise cells to counter-clockwise (reverse the ring) and count them.
+__global__ void k_orient_cells(double *__restrict__ cells, int64_t ncells, int nv,
+                               int *__restrict__ nflipped) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    double *p = cells + 3 * nv * c;
+    if (ring_area(p, nv) >= 0.0) return;
+    for (int a = 1, b = nv - 1; a < b; ++a, --b)
+        for (int d = 0; d < 3; ++d) {
+            double t = p[3 * a + d];
+            p[3 * a + d] = p[3 * b + d];
+            p[3 * b + d] = t;
+        }
+    atomicAdd(nflipped, 1);
+}
+
+__global__ void k_cell_areas(const double *__restrict__ cells, int64_t ncells, int nv,
+                             double *__restrict__ area) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    area[c] = ring_area(cells + 3 * nv * c, nv);
+}
+
+void orient_grid(cr_grid *g, hipStream_t s) {
+    DevBuf<int> cnt(1);
+    CR_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_orient_cells, dim3(grid_for(g->ncells, 256)), dim3(256), 0, s, g->cells.p,
+                       g->ncells, g->nv, cnt.p);
+    CR_HIP(hipGetLastError());
+    int h = 0;
+    CR_HIP(hipMemcpyAsync(&h, cnt.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    CR_HIP(hipStreamSynchronize(s));
+    g->flipped = h;
+}
+
+template <class F>
+void parallel_for(int64_t n, F &&f) {
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 4096) nt = 1;
+    std::vector<std::thread> th;
+    int64_t chunk = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        int64_t a = t * chunk, b = std::min<int64_t>(n, a + chunk);
+        if (a >= b) break;
+        th.emplace_back([=, &f] {
+            for (int64_t i = a; i < b; ++i) f(i);
+        });
+    }
+    for (auto &x : th) x.join();
+}
+
+}  // namespace cr
+
+using namespace cr;
+
+extern "C" {
+
+cr_status cr_gen_lonlat_points(double *xyz, int64_t ni, int64_t nj, double lon0, double lat0,
+                               double dlon, double dlat) {
+    return guard([&] {
+        CR_REQUIRE(xyz && ni > 0 && nj > 0, CR_INVALID_ARGUMENT, "cr_gen_lonlat_points: bad arguments");
+        double lat_end = lat0 + (double)nj * dlat;
+        CR_REQUIRE(lat0 >= -90.0 && lat0 <= 90.0 && lat_end >= -90.0 && lat_end <= 90.0,
+                   CR_INVALID_ARGUMENT, "latitudes outside [-90, 90]");
+        int64_t W = ni + 1;
+        parallel_for((nj + 1) * W, [&](int64_t t) {
+            int64_t j = t / W, i = t - j * W;
+            lonlat_to_xyz(lon0 + (double)i * dlon, lat0 + (double)j * dlat, xyz + 3 * t);
+        });
+    });
+}
+
+cr_status cr_gen_cubed_sphere_points(double *xyz, int64_t ne) {
+    return guard([&] {
+        CR_REQUIRE(xyz && ne > 0, CR_INVALID_ARGUMENT, "cr_gen_cubed_sphere_points: bad arguments");
+        int64_t W = ne + 1;
+        parallel_for(6 * W * W, [&](int64_t t) {
+            int64_t f = t / (W * W), r = t - f * W * W;
+            int64_t j = r / W, i = r - j * W;
+            cube_point((int)f, i, j, ne, xyz + 3 * t);
+        });
+    });
+}
+
+cr_status cr_gen_healpix_cells(double *xyz, int64_t nside, int32_t order, int32_t step,
+                               int64_t first, int64_t count) {
+    return guard([&] {
+        CR_REQUIRE(xyz && nside > 0 && step > 0, CR_INVALID_ARGUMENT, "cr_gen_healpix_cells: bad arguments");
+        CR_REQUIRE(order == CR_HEALPIX_NESTED || order == CR_HEALPIX_RING, CR_INVALID_ARGUMENT,
+                   "HEALPix order must be NESTED or RING");
+        CR_REQUIRE((order == CR_HEALPIX_RING) || ((nside & (nside - 1)) == 0), CR_INVALID_ARGUMENT,
+                   "nested HEALPix ordering needs nside to be a power of 2");
+        CR_REQUIRE(nside <= (1 << 20), CR_INVALID_ARGUMENT, "nside too large");
+        int64_t npix = 12 * nside * nside;
+        CR_REQUIRE(first >= 0 && count >= 0 && first + count <= npix, CR_INVALID_ARGUMENT,
+                   "HEALPix cell range out of bounds");
+        int nv = 4 * step;
+        parallel_for(count, [&](int64_t c) {
+            int f;
+            int64_t ix, iy;
+            hpx_cell_fxy(nside, order, first + c, &f, &ix, &iy);
+            for (int k = 0; k < nv; ++k) hpx_pixel_vertex(f, ix, iy, nside, step, k, xyz + 3 * (c * nv + k));
+        });
+    });
+}
+
+cr_status cr_healpix_nest2ring(int64_t nside, int64_t nest, int64_t *ring) {
+    return guard([&] {
+        CR_REQUIRE(nside > 0 && (nside & (nside - 1)) == 0 && ring, CR_INVALID_ARGUMENT, "bad nside");
+        CR_REQUIRE(nest >= 0 && nest < 12 * nside * nside, CR_INVALID_ARGUMENT, "pixel out of range");
+        int f;
+        int64_t ix, iy;
+        hpx_nest2fxy(nside, nest, &f, &ix, &iy);
+        *ring = hpx_fxy2ring(nside, f, ix, iy);
+    });
+}
+
+cr_status cr_healpix_ring2nest(int64_t nside, int64_t ring, int64_t *nest) {
+    return guard([&] {
+        CR_REQUIRE(nside > 0 && (nside & (nside - 1)) == 0 && nest, CR_INVALID_ARGUMENT, "bad nside");
+        CR_REQUIRE(ring >= 0 && ring < 12 * nside * nside, CR_INVALID_ARGUMENT, "pixel out of range");
+        int f;
+        int64_t ix, iy;
+        hpx_ring2fxy(nside, ring, &f, &ix, &iy);
+        *nest = hpx_fxy2nest(nside, f, ix, iy);
+    });
+}
+
+cr_status cr_points_to_cells(const double *pts, int64_t nblocks, int64_t ni, int64_t nj,
+                             double *cells) {
+    return guard([&] {
+        CR_REQUIRE(pts && cells && nblocks > 0 && ni > 0 && nj > 0, CR_INVALID_ARGUMENT,
+                   "cr_points_to_cells: bad arguments");
+        int64_t per = ni * nj, W = ni + 1;
+        parallel_for(nblocks * per, [&](int64_t c) {
+            int64_t b = c / per, r = c - b * per;
+            int64_t j = r / ni, i = r - j * ni;
+            const double *P = pts + 3 * b * W * (nj + 1);
+            int64_t idx[4] = {i + j * W, i + 1 + j * W, i + 1 + (j + 1) * W, i + (j + 1) * W};
+            for (int k = 0; k < 4; ++k)
+                for (int d = 0; d < 3; ++d) cells[(c * 4 + k) * 3 + d] = P[idx[k] * 3 + d];
+        });
+    });
+}
+
+cr_status cr_grid_from_cells(cr_grid **grid, const double *cells, int64_t ncells, int32_t nv,
+                             int32_t convex, void *stream) {
+    return guard([&] {
+        CR_REQUIRE(grid && cells && ncells > 0 && nv >= 3 && nv <= 64, CR_INVALID_ARGUMENT,
+                   "cr_grid_from_cells: need ncells > 0 and 3 <= nv <= 64");
+        CR_REQUIRE(ncells < (int64_t(1) << 31), CR_UNSUPPORTED, "more than 2^31-1 cells");
+        check_device();
+        hipStream_t s = as_stream(stream);
+        auto *g = new cr_grid();
+        try {
+            g->ncells = ncells;
+            g->nv = nv;
+            g->convex = convex ? 1 : 0;
+            g->cells.alloc((size_t)ncells * nv * 3);
+            CR_HIP(hipMemcpyAsync(g->cells.p, cells, g->cells.bytes(), hipMemcpyHostToDevice, s));
+            orient_grid(g, s);
+        } catch (...) {
+            delete g;
+            throw;
+        }
+        *grid = g;
+    });
+}
+
+cr_status cr_grid_from_points(cr_grid **grid, const double *pts, int64_t nblocks, int64_t ni,
+                              int64_t nj, void *stream) {
+    return guard([&] {
+        CR_REQUIRE(grid && pts && nblocks > 0 && ni > 0 && nj > 0, CR_INVALID_ARGUMENT,
+                   "cr_grid_from_points: bad arguments");
+        CR_REQUIRE(nblocks * ni * nj < (int64_t(1) << 31), CR_UNSUPPORTED, "more than 2^31-1 cells");
+        check_device();
+        hipStream_t s = as_stream(stream);
+        int64_t npts = nblocks * (ni + 1) * (nj + 1);
+        DevBuf<double> dpts((size_t)npts * 3);
+        CR_HIP(hipMemcpyAsync(dpts.p, pts, dpts.bytes(), hipMemcpyHostToDevice, s));
+        auto *g = new cr_grid();
+        try {
+            g->ncells = nblocks * ni * nj;
+            g->nv = 4;
+            g->convex = 1;
+            g->cells.alloc((size_t)g->ncells * 4 * 3);
+            hipLaunchKernelGGL(k_points_to_cells, dim3(grid_for(g->ncells, 256)), dim3(256), 0, s,
+                               dpts.p, nblocks, ni, nj, g->cells.p);
+            CR_HIP(hipGetLastError());
+            orient_grid(g, s);
+        } catch (...) {
+            delete g;
+            throw;
+        }
+        *grid = g;
+    });
+}
+
+cr_status cr_grid_healpix(cr_grid **grid, int64_t nside, int32_t order, int32_t step, void *stream) {
+    return guard([&] {
+        CR_REQUIRE(grid && nside > 0 && step > 0 && step <= 16, CR_INVALID_ARGUMENT,
+                   "cr_grid_healpix: need nside > 0 and 1 <= step <= 16");
+        CR_REQUIRE(order == CR_HEALPIX_NESTED || order == CR_HEALPIX_RING, CR_INVALID_ARGUMENT,
+                   "HEALPix order must be NESTED or RING");
+        CR_REQUIRE((order == CR_HEALPIX_RING) || ((nside & (nside - 1)) == 0), CR_INVALID_ARGUMENT,
+                   "nested HEALPix ordering needs nside to be a power of 2");
+        CR_REQUIRE(nside <= 8192, CR_UNSUPPORTED, "nside above 8192");
+        check_device();
+        hipStream_t s = as_stream(stream);
+        auto *g = new cr_grid();
+        try {
+            g->ncells = 12 * nside * nside;
+            g->nv = 4 * step;
+            g->convex = step == 1 ? 1 : 0;
+            g->cells.alloc((size_t)g->ncells * g->nv * 3);
+            int64_t nthr = g->ncells * g->nv;
+            hipLaunchKernelGGL(k_healpix_cells, dim3(grid_for(nthr, 256)), dim3(256), 0, s, nside, order,
+                               step, int64_t(0), g->ncells, g->cells.p);
+            CR_HIP(hipGetLastError());
+            CR_HIP(hipStreamSynchronize(s));
+        } catch (...) {
+            delete g;
+            throw;
+        }
+        *grid = g;
+    });
+}
+
+cr_status cr_grid_info(const cr_grid *grid, int64_t *ncells, int32_t *nv, int32_t *convex) {
+    return guard([&] {
+        CR_REQUIRE(grid, CR_INVALID_ARGUMENT, "null grid");
+        if (ncells) *ncells = grid->ncells;
+        if (nv) *nv = grid->nv;
+        if (convex) *convex = grid->convex;
+    });
+}
+
+cr_status cr_grid_copy_cells(const cr_grid *grid, double *cells) {
+    return guard([&] {
+        CR_REQUIRE(grid && cells, CR_INVALID_ARGUMENT, "null argument");
+        CR_HIP(hipMemcpy(cells, grid->cells.p, grid->cells.bytes(), hipMemcpyDeviceToHost));
+    });
+}
+
+cr_status cr_grid_areas(const cr_grid *grid, double *areas_host) {
+    return guard([&] {
+        CR_REQUIRE(grid && areas_host, CR_INVALID_ARGUMENT, "null argument");
+        DevBuf<double> a(grid->ncells);
+        hipLaunchKernelGGL(k_cell_areas, dim3(grid_for(grid->ncells, 256)), dim3(256), 0, 0,
+                           grid->cells.p, grid->ncells, grid->nv, a.p);
+        CR_HIP(hipGetLastError());
+        CR_HIP(hipMemcpy(areas_host, a.p, a.bytes(), hipMemcpyDeviceToHost));
+    });
+}
+
+void cr_grid_destroy(cr_grid *grid) { delete grid; }
+
+}  // extern "C"

@@ -1,0 +1,48 @@
+// regridder.hpp -- the device-resident Regridder handle.
+//
+// Mirrors the fields of the reference struct (src/regridder/regridder.jl:25-36):
+//   intersections  -> CSR(A) (rows = dst cells) and CSR(A^T) (rows = src cells);
+//                     the transpose is materialised once so transpose(R) stays a
+//                     flag and both directions are gather SpMVs without atomics
+//   dst_areas      -> a_dst
+//   src_areas      -> a_src
+//   dst_temp/src_temp (work arrays for strided fields) have no device
+//   counterpart: cr_regrid takes strides.
+#pragma once
+
+#include "common.hpp"
+
+namespace cr {
+
+// Row-block plan of one CSR for the batched apply: block b covers rows
+// [row0[b], row0[b+1]).  Blocks hold at most kPlanRows rows and, unless a
+// single row is longer, at most kPlanNnz non-zeros.
+struct ApplyPlan {
+    int64_t nblocks = 0;
+    DevBuf<int32_t> row0;  // nblocks + 1
+    int32_t max_block_nnz = 0;
+};
+
+struct Csr {
+    int64_t nrows = 0, ncols = 0, nnz = 0;
+    DevBuf<int64_t> rowptr;
+    DevBuf<int32_t> col;
+    DevBuf<double> val;
+    DevBuf<double> inv_area;  // 1 / area of the row cells (FP32 batched path)
+    ApplyPlan plan;
+};
+
+}  // namespace cr
+
+struct cr_regridder {
+    cr::Csr fwd;  // A:   n_dst x n_src
+    cr::Csr bwd;  // A^T: n_src x n_dst
+    cr::DevBuf<double> a_dst, a_src;
+    cr_build_stats stats{};
+};
+
+namespace cr {
+// Finish a regridder whose CSR arrays and areas are filled: inverse areas and
+// apply plans.
+void finalize_regridder(cr_regridder *R, hipStream_t s);
+}  // namespace cr

@@ -1,0 +1,172 @@
+/*
+ * crhip.h -- C ABI of the MI355X (gfx950) conservative-regridding library.
+ *
+ * The library replaces what sits behind ConservativeRegridding.jl's two hot
+ * stages (paths relative to the reference repository root):
+ *
+ *   build   Regridder(dst, src; normalize, ...)       src/regridder/regridder.jl:105-163
+ *             intersection_areas(manifold, threaded, dst_tree, src_tree; ...)
+ *                                                     src/regridder/intersection_areas.jl:67-157
+ *             areas(manifold, item, tree)             src/regridder/regridder.jl:165-178
+ *             normalize!(regridder)                   src/regridder/regridder.jl:54-62
+ *   apply   regrid!(dst, R, src; dims=1, normalize)   src/regridder/regrid.jl:63-118
+ *             NDSliceLoop batched slices              src/regridder/regrid.jl:303-318
+ *             transpose(R)                            src/regridder/regridder.jl:49-50
+ *
+ * Conventions
+ *   - Every function returns a cr_status; cr_last_error() gives a thread-local
+ *     message for the last failing call on the calling thread.
+ *   - Cell and point coordinates are unit 3-vectors stored xyz-interleaved as
+ *     double.  Cell indices are 0-based (the Julia layer adds 1).
+ *   - A polygon table is (ncells x nv) vertices, open rings (no repeated closing
+ *     vertex), counter-clockwise seen from outside the sphere.  Cells with fewer
+ *     than nv vertices repeat their last vertex.
+ *   - Rows of A are destination cells and columns are source cells
+ *     (intersection_areas.jl:115-121).
+ *   - `stream` is a hipStream_t passed as void*; NULL is the null stream.
+ *   - Handles are immutable after creation except through cr_regridder_normalize;
+ *     concurrent cr_regrid calls on one handle with distinct outputs are safe.
+ */
+#ifndef CRHIP_H
+#define CRHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    CR_OK = 0,
+    CR_INVALID_ARGUMENT = 1,   /* Julia: ArgumentError                         */
+    CR_DIMENSION_MISMATCH = 2, /* Julia: DimensionMismatch (regrid.jl:289-296) */
+    CR_UNSUPPORTED = 3,        /* Julia: MethodError / ErrorException          */
+    CR_OUT_OF_MEMORY = 4,
+    CR_HIP_ERROR = 5,
+    CR_DEGENERATE_GEOMETRY = 6,
+    CR_INTERNAL = 7,
+    CR_NO_DEVICE = 8
+} cr_status;
+
+typedef enum { CR_F64 = 0, CR_F32 = 1 } cr_dtype;
+typedef enum { CR_HEALPIX_NESTED = 0, CR_HEALPIX_RING = 1 } cr_healpix_order;
+typedef enum { CR_CLIP_AUTO = 0, CR_CLIP_WITH_DST = 1, CR_CLIP_WITH_SRC = 2 } cr_clip_side;
+
+typedef struct cr_grid cr_grid;
+typedef struct cr_regridder cr_regridder;
+
+/* Build options (keyword arguments of Regridder, regridder.jl:125-131). */
+typedef struct {
+    double threshold_rel; /* drop A_ij <= threshold_rel * min(a_src_j, a_dst_i);
+                             default 1e-14; 0 keeps every area > 0 like
+                             intersection_areas.jl:24 */
+    double radius;        /* sphere radius; areas are in radius^2 units
+                             (default 6371008.8, GeometryOpsCore's Spherical()) */
+    int32_t normalize;    /* Regridder(...; normalize=true): normalize!(R) */
+    int32_t clip_side;    /* cr_clip_side; AUTO clips with a convex side */
+    int32_t reserved0;
+    int32_t reserved1;
+} cr_build_opts;
+
+/* Per-build counters and stage times (device event timing, milliseconds). */
+typedef struct {
+    int64_t n_dst, n_src, nnz, n_candidates;
+    double ms_total, ms_bounds, ms_tree, ms_search, ms_clip, ms_assemble;
+    int32_t clip_with_dst; /* 1: dst cells were the clip polygons */
+    int32_t flipped_cells; /* cells re-oriented to CCW at grid creation */
+} cr_build_stats;
+
+/* ------------------------------------------------------------ library -- */
+const char *cr_last_error(void);
+const char *cr_version(void);
+/* Number of visible HIP devices (0 on a host without a GPU; never fails). */
+int32_t cr_device_count(void);
+
+/* ------------------------------------------- grid generators (host only) -- */
+/* These replace the grid packages the reference's adapters call into; they
+ * need no GPU.  Point matrices are (ni+1) x (nj+1), i fastest, the layout of
+ * the Matrix{UnitSphericalPoint} that Trees.treeify turns into a CellBasedGrid
+ * (src/trees/interfaces.jl:125-126, src/trees/grids.jl:57-85). */
+
+/* Lon-lat points lon0 + i*dlon, lat0 + j*dlat (degrees), converted with exact
+ * degree sincos (poles and multiples of 90 degrees are exact). */
+cr_status cr_gen_lonlat_points(double *xyz, int64_t ni, int64_t nj, double lon0_deg,
+                               double lat0_deg, double dlon_deg, double dlat_deg);
+/* Equiangular gnomonic cubed sphere: 6 faces of (ne+1)^2 points, face-major.
+ * Face f covers cells f*ne^2 + i + j*ne (the offsets of
+ * ext/ConservativeRegriddingClimaCoreExt.jl:56-63). */
+cr_status cr_gen_cubed_sphere_points(double *xyz, int64_t ne);
+/* HEALPix cells [first, first+count) as 4*step-vertex polygons, counter-
+ * clockwise from the north corner, in nested or ring index order.  Replaces
+ * Healpix.boundariesRing (ext/ConservativeRegriddingHealpixExt.jl:76-90). */
+cr_status cr_gen_healpix_cells(double *xyz, int64_t nside, int32_t order, int32_t step,
+                               int64_t first, int64_t count);
+/* HEALPix index conversions (0-based).  Healpix.nest2ring / ring2nest. */
+cr_status cr_healpix_nest2ring(int64_t nside, int64_t nest, int64_t *ring);
+cr_status cr_healpix_ring2nest(int64_t nside, int64_t ring, int64_t *nest);
+/* Cell polygons of point blocks: nblocks blocks of (ni+1)*(nj+1) points; cell
+ * (b,i,j) -> index b*ni*nj + i + j*ni with vertices P[i,j], P[i+1,j],
+ * P[i+1,j+1], P[i,j+1] (CellBasedGrid getcell, grids.jl:71-84). */
+cr_status cr_points_to_cells(const double *pts, int64_t nblocks, int64_t ni, int64_t nj,
+                             double *cells);
+
+/* ------------------------------------------------------ device grids -- */
+/* Upload a host polygon table.  `convex` declares every cell convex. */
+cr_status cr_grid_from_cells(cr_grid **grid, const double *cells, int64_t ncells, int32_t nv,
+                             int32_t convex, void *stream);
+/* Upload point blocks (a lon-lat Matrix{UnitSphericalPoint} is one block, a
+ * cubed sphere six) and expand them to cells on the device. */
+cr_status cr_grid_from_points(cr_grid **grid, const double *pts, int64_t nblocks, int64_t ni,
+                              int64_t nj, void *stream);
+/* Generate a HEALPix grid on the device (HealpixRootNode{O}(nside)). */
+cr_status cr_grid_healpix(cr_grid **grid, int64_t nside, int32_t order, int32_t step,
+                          void *stream);
+cr_status cr_grid_info(const cr_grid *grid, int64_t *ncells, int32_t *nv, int32_t *convex);
+/* Download the (re-oriented) polygon table, ncells*nv*3 doubles. */
+cr_status cr_grid_copy_cells(const cr_grid *grid, double *cells);
+/* Cell areas on the unit sphere, computed on the device (areas(), regridder.jl:165-178). */
+cr_status cr_grid_areas(const cr_grid *grid, double *areas_host);
+void cr_grid_destroy(cr_grid *grid);
+
+/* ------------------------------------------------------------- build -- */
+void cr_build_opts_default(cr_build_opts *opts);
+/* Regridder(dst, src; kwargs...) on the device: candidate search, FP64 clip and
+ * area per pair, CSR(A), CSR(A^T), cell areas, optional normalize!. */
+cr_status cr_regridder_build(cr_regridder **R, const cr_grid *dst, const cr_grid *src,
+                             const cr_build_opts *opts, void *stream);
+/* Regridder(intersections, dst_areas, src_areas, ...) from host CSR arrays
+ * (the struct's field constructor, regridder.jl:25-36). */
+cr_status cr_regridder_from_csr(cr_regridder **R, int64_t n_dst, int64_t n_src, int64_t nnz,
+                                const int64_t *rowptr, const int32_t *col, const double *val,
+                                const double *dst_areas, const double *src_areas, void *stream);
+cr_status cr_regridder_info(const cr_regridder *R, int64_t *n_dst, int64_t *n_src, int64_t *nnz);
+cr_status cr_regridder_stats(const cr_regridder *R, cr_build_stats *stats);
+/* Host copies of CSR(A) (transpose=0) or CSR(A^T) (transpose=1). */
+cr_status cr_regridder_copy_csr(const cr_regridder *R, int32_t transpose, int64_t *rowptr,
+                                int32_t *col, double *val);
+cr_status cr_regridder_copy_areas(const cr_regridder *R, double *dst_areas, double *src_areas);
+/* normalize!(R): A, a_src, a_dst ./= maximum(A) (regridder.jl:54-62). */
+cr_status cr_regridder_normalize(cr_regridder *R, void *stream);
+void cr_regridder_destroy(cr_regridder *R);
+
+/* ------------------------------------------------------------- apply -- */
+/*
+ * regrid!(dst, R, src) for nfields fields at once (transpose=1: transpose(R)).
+ * src and dst are DEVICE pointers of dtype elements.  Element (cell c, field f)
+ * is at ptr[c*cell_stride + f*field_stride].  cells-fastest ([n,64,4] with
+ * dims=1) is cell_stride=1, field_stride=n; fields-fastest is cell_stride=k,
+ * field_stride=1.  normalize=1 divides by the destination areas
+ * (finalize_regridding!, regrid.jl:104-118).  CR_F64 fields accumulate in FP64;
+ * CR_F32 fields accumulate in FP32 with Kahan-compensated row sums against the
+ * FP64 A.  The forward and transpose paths are gathers (no atomics): results
+ * are bitwise reproducible.
+ */
+cr_status cr_regrid(const cr_regridder *R, int32_t transpose, int32_t dtype, int64_t nfields,
+                    const void *src, int64_t src_cell_stride, int64_t src_field_stride,
+                    void *dst, int64_t dst_cell_stride, int64_t dst_field_stride,
+                    int32_t normalize, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CRHIP_H */

@@ -1,0 +1,199 @@
+"""ctypes front-end of the CPU oracle (oracle/cr_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` -- never by the product package.
+
+The functions mirror the reference's CPU path (see the header of cr_oracle.c
+for the file:line map):
+
+* :func:`build`      -- ``Regridder(dst, src)`` (regridder.jl:125-163)
+* :func:`spmv`       -- ``regrid!`` single field (regrid.jl:63-118)
+* :func:`transpose`  -- the CSR of ``transpose(R).intersections`` (regridder.jl:49-50)
+* :func:`clip_area`, :func:`polygon_area`, :func:`cell_areas` -- the per-pair
+  operator (regridder.jl:96-103) and ``areas`` (regridder.jl:165-178)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+
+
+def build_library() -> str:
+    """Compile oracle/build/liboracle.so with gcc (make -C oracle)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build_library()
+    L = ctypes.CDLL(_LIB_PATH)
+    L.orc_polygon_area.restype = ctypes.c_double
+    L.orc_polygon_area.argtypes = [_dp, ctypes.c_int32]
+    L.orc_clip_area.restype = ctypes.c_double
+    L.orc_clip_area.argtypes = [_dp, ctypes.c_int32, _dp, ctypes.c_int32]
+    L.orc_clip_polygon.restype = ctypes.c_int32
+    L.orc_clip_polygon.argtypes = [_dp, ctypes.c_int32, _dp, ctypes.c_int32, _dp, ctypes.c_int32]
+    L.orc_cell_areas.restype = None
+    L.orc_cell_areas.argtypes = [_dp, ctypes.c_int64, ctypes.c_int32, _dp, ctypes.c_int32]
+    L.orc_build.restype = ctypes.c_void_p
+    L.orc_build.argtypes = [_dp, ctypes.c_int64, ctypes.c_int32, _dp, ctypes.c_int64, ctypes.c_int32,
+                            ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64]
+    for name in ("orc_result_nnz", "orc_result_candidates"):
+        getattr(L, name).restype = ctypes.c_int64
+        getattr(L, name).argtypes = [ctypes.c_void_p]
+    L.orc_result_status.restype = ctypes.c_int32
+    L.orc_result_status.argtypes = [ctypes.c_void_p]
+    L.orc_result_times.restype = None
+    L.orc_result_times.argtypes = [ctypes.c_void_p, _dp]
+    L.orc_result_copy.restype = None
+    L.orc_result_copy.argtypes = [ctypes.c_void_p, _i64p, _i32p, _dp, _dp, _dp]
+    L.orc_result_free.restype = None
+    L.orc_result_free.argtypes = [ctypes.c_void_p]
+    L.orc_spmv.restype = None
+    L.orc_spmv.argtypes = [_i64p, _i32p, _dp, ctypes.c_int64, _dp, _dp, _dp, ctypes.c_int32]
+    L.orc_transpose.restype = None
+    L.orc_transpose.argtypes = [_i64p, _i32p, _dp, ctypes.c_int64, ctypes.c_int64, _i64p, _i32p, _dp]
+    _lib = L
+    return L
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def polygon_area(ring) -> float:
+    """Signed solid angle of an open ring of unit vectors, shape (n, 3)."""
+    r = _f64(ring)
+    return lib().orc_polygon_area(_d(r), r.shape[0])
+
+
+def clip_area(subject, clip) -> float:
+    """Area of subject ∩ clip (clip convex, both CCW open rings of unit vectors)."""
+    s, c = _f64(subject), _f64(clip)
+    return lib().orc_clip_area(_d(s), s.shape[0], _d(c), c.shape[0])
+
+
+def clip_polygon(subject, clip) -> np.ndarray:
+    s, c = _f64(subject), _f64(clip)
+    out = np.zeros((256, 3))
+    m = lib().orc_clip_polygon(_d(s), s.shape[0], _d(c), c.shape[0], _d(out), 256)
+    if m < 0:
+        raise RuntimeError("clip buffer overflow")
+    return out[:m].copy()
+
+
+def cell_areas(cells, nthreads: int = 0) -> np.ndarray:
+    """Areas of a polygon table, shape (ncells, nv, 3)."""
+    c = _f64(cells)
+    out = np.empty(c.shape[0])
+    lib().orc_cell_areas(_d(c), c.shape[0], c.shape[1], _d(out), nthreads or os.cpu_count() or 1)
+    return out
+
+
+@dataclass
+class OracleCSR:
+    n_dst: int
+    n_src: int
+    rowptr: np.ndarray
+    col: np.ndarray
+    val: np.ndarray
+    a_dst: np.ndarray
+    a_src: np.ndarray
+    n_candidates: int
+    seconds: float
+    threads: int
+
+    @property
+    def nnz(self) -> int:
+        return int(self.rowptr[-1])
+
+    def row_index(self) -> np.ndarray:
+        return np.repeat(np.arange(self.n_dst, dtype=np.int64), np.diff(self.rowptr))
+
+    def transpose(self) -> "OracleCSR":
+        rp, c, v = transpose(self.rowptr, self.col, self.val, self.n_dst, self.n_src)
+        return OracleCSR(self.n_src, self.n_dst, rp, c, v, self.a_src, self.a_dst,
+                         self.n_candidates, self.seconds, self.threads)
+
+    def apply(self, x, normalize: bool = True, nthreads: int = 1) -> np.ndarray:
+        return spmv(self.rowptr, self.col, self.val, x, self.a_dst if normalize else None, nthreads)
+
+    def normalized(self) -> "OracleCSR":
+        """normalize!(R): A, a_s, a_d ./= maximum(A)  (regridder.jl:54-62)."""
+        m = self.val.max()
+        return OracleCSR(self.n_dst, self.n_src, self.rowptr, self.col, self.val / m,
+                         self.a_dst / m, self.a_src / m, self.n_candidates, self.seconds, self.threads)
+
+
+def build(dst_cells, src_cells, clip_is_dst: bool = True, threshold_rel: float = 1e-14,
+          nthreads: int = 0, row_range=None) -> OracleCSR:
+    """Regridder build on the unit sphere.  ``dst_cells``/``src_cells`` are
+    polygon tables (ncells, nv, 3).  ``clip_is_dst`` picks the convex side."""
+    d, s = _f64(dst_cells), _f64(src_cells)
+    nth = nthreads or (os.cpu_count() or 1)
+    r0, r1 = (0, d.shape[0]) if row_range is None else row_range
+    L = lib()
+    h = L.orc_build(_d(d), d.shape[0], d.shape[1], _d(s), s.shape[0], s.shape[1],
+                    1 if clip_is_dst else 0, threshold_rel, nth, r0, r1)
+    try:
+        st = L.orc_result_status(h)
+        if st != 0:
+            raise RuntimeError(f"oracle build failed (status {st})")
+        nnz = L.orc_result_nnz(h)
+        rowptr = np.empty(d.shape[0] + 1, np.int64)
+        col = np.empty(nnz, np.int32)
+        val = np.empty(nnz, np.float64)
+        a_dst = np.empty(d.shape[0])
+        a_src = np.empty(s.shape[0])
+        L.orc_result_copy(h, rowptr.ctypes.data_as(_i64p), col.ctypes.data_as(_i32p), _d(val),
+                          _d(a_dst), _d(a_src))
+        t = np.zeros(3)
+        L.orc_result_times(h, _d(t))
+        ncand = L.orc_result_candidates(h)
+    finally:
+        L.orc_result_free(h)
+    return OracleCSR(d.shape[0], s.shape[0], rowptr, col, val, a_dst, a_src, ncand, float(t[0]), nth)
+
+
+def spmv(rowptr, col, val, x, area=None, nthreads: int = 1) -> np.ndarray:
+    rowptr = np.ascontiguousarray(rowptr, np.int64)
+    col = np.ascontiguousarray(col, np.int32)
+    val, x = _f64(val), _f64(x)
+    n = rowptr.shape[0] - 1
+    y = np.empty(n)
+    a = _f64(area) if area is not None else None
+    lib().orc_spmv(rowptr.ctypes.data_as(_i64p), col.ctypes.data_as(_i32p), _d(val), n, _d(x), _d(y),
+                   _d(a) if a is not None else None, nthreads)
+    return y
+
+
+def transpose(rowptr, col, val, nrows: int, ncols: int):
+    rowptr = np.ascontiguousarray(rowptr, np.int64)
+    col = np.ascontiguousarray(col, np.int32)
+    val = _f64(val)
+    trp = np.empty(ncols + 1, np.int64)
+    tcol = np.empty(col.shape[0], np.int32)
+    tval = np.empty(col.shape[0])
+    lib().orc_transpose(rowptr.ctypes.data_as(_i64p), col.ctypes.data_as(_i32p), _d(val), nrows, ncols,
+                        trp.ctypes.data_as(_i64p), tcol.ctypes.data_as(_i32p), _d(tval))
+    return trp, tcol, tval

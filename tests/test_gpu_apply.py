@@ -1,0 +1,196 @@
+"""Device regrid! (forward and transpose(R)) vs the CPU oracle and the README
+golden vectors; batched FP32 vs FP64; the N-D / strided / host-array pipeline
+of src/regridder/regrid.jl (test/regridding.jl:45-203 semantics)."""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+DEV = "cuda"
+
+
+def _case(name, **kw):
+    print(f"[case] {name} {kw}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def c2(cr, orc):
+    sp_, cp = cr.lonlat_points(360, 180), cr.cubed_sphere_points(48)
+    R = cr.Regridder(cr.Grid.from_points(cp), cr.Grid.from_points(sp_), radius=1.0)
+    rp, col, val = R.csr_arrays()
+    return R, sp.csr_matrix((val, col, rp), shape=R.shape)
+
+
+def test_readme_golden_vectors(cr):
+    _case("readme")
+    d = json.load(open(os.path.join(GOLDEN, "readme_example.json")))
+    R = cr.Regridder.from_csr(sp.csr_matrix(np.array(d["A_dense"])), d["dst_areas"], d["src_areas"])
+    orange = torch.tensor(d["orange"], dtype=torch.float64, device=DEV)
+    blue = torch.zeros(4, dtype=torch.float64, device=DEV)
+    cr.regrid_(blue, R, orange)
+    a_s, a_d = np.array(d["src_areas"]), np.array(d["dst_areas"])
+    o, b = orange.cpu().numpy(), blue.cpu().numpy()
+    assert (o * a_s).sum() / a_s.sum() == (b * a_d).sum() / a_d.sum()
+    tilde = torch.zeros(5, dtype=torch.float64, device=DEV)
+    T = cr.transpose(R)
+    assert T.src_areas is R.dst_areas and T.dst_areas is R.src_areas
+    cr.regrid_(tilde, T, blue)
+    np.testing.assert_allclose(tilde.cpu().numpy(), d["orange_tilde"], rtol=2e-16, atol=0)
+
+
+def test_fp64_forward_and_transpose_match_oracle(cr, orc, c2):
+    _case("fp64_c2")
+    R, A = c2
+    rng = np.random.default_rng(0)
+    s = rng.random(A.shape[1])
+    ref = orc.spmv(A.indptr, A.indices, A.data, s, R.dst_areas)
+    d = torch.zeros(A.shape[0], dtype=torch.float64, device=DEV)
+    cr.regrid_(d, R, torch.tensor(s, device=DEV))
+    got = d.cpu().numpy()
+    assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+    # conservation, checked on every run
+    assert abs((got * R.dst_areas).sum() - (s * R.src_areas).sum()) <= 1e-12 * (s * R.src_areas).sum()
+    trp, tcol, tval = orc.transpose(A.indptr, A.indices, A.data, A.shape[0], A.shape[1])
+    back_ref = orc.spmv(trp, tcol, tval, got, R.src_areas)
+    back = torch.zeros(A.shape[1], dtype=torch.float64, device=DEV)
+    cr.regrid_(back, cr.transpose(R), d)
+    assert np.max(np.abs(back.cpu().numpy() - back_ref)) <= 1e-12 * np.max(np.abs(back_ref))
+
+
+def test_normalize_false_returns_raw_product(cr, c2):
+    _case("raw")
+    R, A = c2
+    s = np.random.default_rng(1).random(A.shape[1])
+    d = torch.zeros(A.shape[0], dtype=torch.float64, device=DEV)
+    cr.regrid_(d, R, torch.tensor(s, device=DEV), normalize=False)
+    np.testing.assert_allclose(d.cpu().numpy(), A @ s, rtol=1e-13)
+
+
+def _fields(n, k, seed):
+    rng = np.random.default_rng(seed)
+    base = rng.random(k)[None, :] + 0.01 * rng.random((n, k))
+    return base
+
+
+@pytest.mark.parametrize("layout", ["cells_fastest", "fields_fastest"])
+def test_fp32_batched_matches_fp64(cr, c2, layout):
+    _case("fp32_batched", layout=layout)
+    R, A = c2
+    n_dst, n_src = A.shape
+    k = 256
+    x64 = _fields(n_src, k, 3)  # (n_src, k)
+    if layout == "cells_fastest":  # Julia [n, 64, 4] with dims=1
+        x = torch.tensor(x64.T.reshape(4, 64, n_src), dtype=torch.float32, device=DEV).permute(2, 1, 0)
+        y = torch.zeros(4, 64, n_dst, dtype=torch.float32, device=DEV).permute(2, 1, 0)
+        cr.regrid_(y, R, x)
+        got = y.permute(2, 1, 0).reshape(k, n_dst).T.cpu().numpy().astype(np.float64)
+        xin = x.permute(2, 1, 0).reshape(k, n_src).T.cpu().numpy().astype(np.float64)
+    else:
+        x = torch.tensor(x64, dtype=torch.float32, device=DEV)
+        y = torch.zeros(n_dst, k, dtype=torch.float32, device=DEV)
+        cr.regrid_(y, R, x)
+        got = y.cpu().numpy().astype(np.float64)
+        xin = x.cpu().numpy().astype(np.float64)
+    ref = (A @ xin) / R.dst_areas[:, None]
+    rel = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    assert rel <= 1e-6, rel
+    # conservation drift of every field
+    tot_s = (xin * R.src_areas[:, None]).sum(0)
+    tot_d = (got * R.dst_areas[:, None]).sum(0)
+    assert np.max(np.abs(tot_d - tot_s) / np.abs(tot_s)) <= 1e-7
+
+
+def test_fp64_batched_equals_loop_of_single_fields(cr, c2):
+    _case("fp64_batched")
+    R, A = c2
+    n_dst, n_src = A.shape
+    x = torch.tensor(_fields(n_src, 8, 5), device=DEV)  # (n_src, 8): fields fastest
+    y = torch.zeros(n_dst, 8, dtype=torch.float64, device=DEV)
+    cr.regrid_(y, R, x)
+    for f in range(8):
+        yf = torch.zeros(n_dst, dtype=torch.float64, device=DEV)
+        cr.regrid_(yf, R, x[:, f].contiguous())
+        assert torch.equal(yf, y[:, f])
+
+
+def test_strided_views_match_dense(cr, c2):
+    """Dense/strided dispatch (test/regridding.jl:45-125): a strided view gives
+    the dense result; here no temp buffers are needed at all."""
+    _case("strided")
+    R, A = c2
+    n_dst, n_src = A.shape
+    s = torch.tensor(np.random.default_rng(2).random(n_src), device=DEV)
+    ref = torch.zeros(n_dst, dtype=torch.float64, device=DEV)
+    cr.regrid_(ref, R, s)
+    big_src = torch.zeros(2 * n_src, dtype=torch.float64, device=DEV)
+    big_src[::2] = s
+    big_dst = torch.zeros(2 * n_dst, dtype=torch.float64, device=DEV)
+    cr.regrid_(big_dst[::2], R, big_src[::2])
+    assert torch.equal(big_dst[::2], ref)
+    assert torch.all(big_dst[1::2] == 0)
+
+
+def test_host_arrays_are_staged(cr, c2):
+    _case("host")
+    R, A = c2
+    s = np.random.default_rng(3).random(A.shape[1])
+    d = np.zeros(A.shape[0])
+    out = cr.regrid_(d, R, s)
+    assert out is d
+    np.testing.assert_allclose(d, (A @ s) / R.dst_areas, rtol=1e-13)
+    d2 = cr.regrid(R, s)
+    np.testing.assert_array_equal(d2, d)
+
+
+@pytest.mark.parametrize("shape_src,shape_dst,dims", [
+    ((None, 3), (None, 3), 1),
+    ((3, None), (3, None), 2),
+    ((None, 3, 2), (None, 3, 2), 1),
+    ((2, None, 3), (2, None, 3), 2),
+    ((3, 2, None), (3, 2, None), 3),
+])
+def test_nd_constant_fields_stay_constant(cr, c2, shape_src, shape_dst, dims):
+    """regrid! on N-D arrays with dims (test/regridding.jl:127-203)."""
+    _case("nd", dims=dims, shape=shape_src)
+    R, A = c2
+    n_dst, n_src = A.shape
+    ss = tuple(n_src if v is None else v for v in shape_src)
+    ds = tuple(n_dst if v is None else v for v in shape_dst)
+    for make in (lambda sh: torch.ones(sh, dtype=torch.float64, device=DEV), lambda sh: np.ones(sh)):
+        src = make(ss)
+        dst = torch.zeros(ds, dtype=torch.float64, device=DEV) if isinstance(src, torch.Tensor) else np.zeros(ds)
+        cr.regrid_(dst, R, src, dims=dims)
+        arr = dst.cpu().numpy() if isinstance(dst, torch.Tensor) else dst
+        np.testing.assert_allclose(arr, 1.0, atol=1e-12)
+
+
+def test_regional_source_leaves_uncovered_cells_zero(cr):
+    """A source that covers part of the sphere: covered destination cells of a
+    constant field are <= 1, uncovered ones are 0 (test/usecases/constant_field.jl:41-77)."""
+    _case("regional")
+    src_pts = cr.lonlat_points(40, 20, lon0=10.0, lat0=-20.0, dlon=1.0, dlat=2.0)
+    R = cr.Regridder(cr.Grid.from_points(cr.lonlat_points(72, 36)), cr.Grid.from_points(src_pts), radius=1.0)
+    d = cr.regrid(R, torch.ones(800, dtype=torch.float64, device=DEV)).cpu().numpy()
+    covered = np.asarray(R.intersections.sum(1)).ravel() > 0
+    assert np.all(d[~covered] == 0.0)
+    assert np.all(d[covered] <= 1.0 + 1e-12) and np.any(d[covered] > 1.0 - 1e-12)
+    # backward: every source cell is fully covered by the global destination
+    back = cr.regrid(cr.transpose(R), torch.ones(R.shape[0], dtype=torch.float64, device=DEV)).cpu().numpy()
+    np.testing.assert_allclose(back, 1.0, atol=1e-12)
+
+
+def test_normalize_in_place_keeps_results(cr):
+    _case("normalize_")
+    R = cr.Regridder(cr.Grid.from_points(cr.lonlat_points(72, 36)), cr.Grid.from_points(cr.lonlat_points(90, 45)))
+    s = torch.tensor(np.random.default_rng(4).random(R.shape[1]), device=DEV)
+    d1 = cr.regrid(R, s)
+    cr.normalize_(R)
+    assert R.intersections.max() == 1.0
+    d2 = cr.regrid(R, s)
+    torch.testing.assert_close(d1, d2, rtol=1e-14, atol=0)

@@ -1,0 +1,140 @@
+"""Device Regridder build vs the CPU oracle (sparsity pattern, areas to 1e-12
+of the cell area, cell areas to 1e-12), on the configurations' grids."""
+import math
+
+import numpy as np
+import pytest
+
+from gpu_common import jittered_cubed_sphere, pattern_and_values
+
+pytestmark = pytest.mark.gpu
+
+
+def _print_case(name, **kw):
+    print(f"[case] {name} {kw}", flush=True)
+
+
+def test_c1_lonlat4_to_lonlat5_normalized(cr, orc):
+    _print_case("c1", src="4deg", dst="5deg", normalize=True)
+    sp_, dp_ = cr.lonlat_points(90, 45), cr.lonlat_points(72, 36)
+    src, dst = cr.Grid.from_points(sp_), cr.Grid.from_points(dp_)
+    R = cr.Regridder(dst, src, normalize=True, radius=1.0)
+    ref = orc.build(cr.points_to_cells(dp_), cr.points_to_cells(sp_)).normalized()
+    err, flips = pattern_and_values(R, ref)
+    assert R.nnz == 12384 and flips == 0
+    assert R.intersections.max() == 1.0
+    st = R.stats()
+    assert st["n_candidates"] >= R.nnz
+
+
+def test_c2_lonlat1_to_c48(cr, orc):
+    _print_case("c2", src="1deg", dst="C48")
+    sp_, cp = cr.lonlat_points(360, 180), cr.cubed_sphere_points(48)
+    R = cr.Regridder(cr.Grid.from_points(cp), cr.Grid.from_points(sp_), radius=1.0)
+    ref = orc.build(cr.points_to_cells(cp), cr.points_to_cells(sp_))
+    err, flips = pattern_and_values(R, ref)
+    A = R.intersections
+    np.testing.assert_allclose(np.asarray(A.sum(1)).ravel(), R.dst_areas, rtol=1e-12)
+    np.testing.assert_allclose(np.asarray(A.sum(0)).ravel(), R.src_areas, rtol=1e-12)
+
+
+def test_lonlat_to_healpix_16gons(cr, orc):
+    _print_case("hpx32_step4", src="2deg", dst="healpix32 step4")
+    sp_ = cr.lonlat_points(180, 90)
+    dst = cr.Grid.healpix(32, "nested", step=4)
+    assert not dst.convex
+    R = cr.Regridder(dst, cr.Grid.from_points(sp_), radius=1.0)
+    assert R.stats()["clip_with_dst"] == 0
+    ref = orc.build(dst.cells(), cr.points_to_cells(sp_), clip_is_dst=False)
+    pattern_and_values(R, ref)
+
+
+def test_healpix_source_to_lonlat(cr, orc):
+    _print_case("hpx_src", src="healpix16 ring step4", dst="3deg")
+    dp_ = cr.lonlat_points(120, 60)
+    src = cr.Grid.healpix(16, "ring", step=4)
+    R = cr.Regridder(cr.Grid.from_points(dp_), src, radius=1.0)
+    assert R.stats()["clip_with_dst"] == 1
+    ref = orc.build(cr.points_to_cells(dp_), src.cells(), clip_is_dst=True)
+    pattern_and_values(R, ref)
+
+
+def test_c4_jittered_cubed_sphere_to_lonlat1(cr, orc):
+    _print_case("c4", src="C96 jittered seed 4", dst="1deg")
+    P = jittered_cubed_sphere(cr, 96, seed=4)
+    dp_ = cr.lonlat_points(360, 180)
+    R = cr.Regridder(cr.Grid.from_points(dp_), cr.Grid.from_points(P), radius=1.0)
+    ref = orc.build(cr.points_to_cells(dp_), cr.points_to_cells(P))
+    pattern_and_values(R, ref)
+    assert R.stats()["flipped_cells"] == 0
+
+
+def test_c3_full_size(cr, orc):
+    _print_case("c3", src="0.25deg", dst="healpix256 nested step4")
+    sp_ = cr.lonlat_points(1440, 720)
+    dst = cr.Grid.healpix(256, "nested", step=4)
+    R = cr.Regridder(dst, cr.Grid.from_points(sp_), radius=1.0)
+    ref = orc.build(dst.cells(), cr.points_to_cells(sp_), clip_is_dst=False)
+    err, flips = pattern_and_values(R, ref)
+    print(f"c3 nnz={R.nnz} err={err:.2e} flips={flips} stats={R.stats()}", flush=True)
+
+
+def test_device_healpix_matches_host_generator(cr):
+    _print_case("hpx_gen")
+    for order in ("nested", "ring"):
+        g = cr.Grid.healpix(16, order, step=4)
+        np.testing.assert_allclose(g.cells(), cr.healpix_cells(16, order, 4), atol=4e-16, rtol=0)
+
+
+def test_clockwise_cells_are_reoriented(cr, orc):
+    _print_case("cw")
+    sp_, dp_ = cr.lonlat_points(36, 18), cr.lonlat_points(24, 12)
+    cells = cr.points_to_cells(sp_)
+    cw = cells[:, ::-1, :].copy()
+    g = cr.Grid.from_cells(cw, convex=True)
+    R1 = cr.Regridder(cr.Grid.from_points(dp_), g, radius=1.0)
+    R2 = cr.Regridder(cr.Grid.from_points(dp_), cr.Grid.from_points(sp_), radius=1.0)
+    assert R1.stats()["flipped_cells"] == cells.shape[0]
+    A1, A2 = R1.intersections, R2.intersections
+    assert (A1 != A2).nnz == 0 or np.max(np.abs((A1 - A2).data)) < 1e-15
+
+
+def test_radius_scales_areas(cr):
+    _print_case("radius")
+    sp_, dp_ = cr.lonlat_points(36, 18), cr.lonlat_points(24, 12)
+    R1 = cr.Regridder(cr.Grid.from_points(dp_), cr.Grid.from_points(sp_), radius=1.0)
+    Re = cr.Regridder(cr.Grid.from_points(dp_), cr.Grid.from_points(sp_))
+    r2 = cr.EARTH_RADIUS ** 2
+    np.testing.assert_allclose(Re.dst_areas, R1.dst_areas * r2, rtol=1e-15)
+    np.testing.assert_allclose(Re.intersections.data, R1.intersections.data * r2, rtol=1e-15)
+    assert Re.src_areas.sum() == pytest.approx(4 * math.pi * r2, rel=1e-13)
+
+
+def test_threshold_zero_keeps_touching_slivers(cr, orc):
+    _print_case("thr0")
+    sp_, dp_ = cr.lonlat_points(90, 45), cr.lonlat_points(72, 36)
+    R0 = cr.Regridder(cr.Grid.from_points(dp_), cr.Grid.from_points(sp_), radius=1.0, threshold_rel=0.0)
+    R = cr.Regridder(cr.Grid.from_points(dp_), cr.Grid.from_points(sp_), radius=1.0)
+    assert R0.nnz >= R.nnz == 12384
+
+
+def test_self_regrid_is_diagonal(cr):
+    _print_case("self")
+    p = cr.lonlat_points(72, 36)
+    g = cr.Grid.from_points(p)
+    R = cr.Regridder(g, g, radius=1.0)
+    A = R.intersections
+    assert R.nnz == g.ncells
+    np.testing.assert_array_equal(A.indices, np.arange(g.ncells))
+    np.testing.assert_allclose(A.diagonal(), R.dst_areas, rtol=1e-14)
+
+
+def test_transposed_csr_is_consistent(cr):
+    _print_case("transpose_csr")
+    sp_, cp = cr.lonlat_points(90, 45), cr.cubed_sphere_points(12)
+    R = cr.Regridder(cr.Grid.from_points(cp), cr.Grid.from_points(sp_), radius=1.0)
+    A = R.intersections
+    At = cr.transpose(R).intersections
+    assert At.shape == (A.shape[1], A.shape[0])
+    assert (At - A.T.tocsr()).nnz == 0 or np.max(np.abs((At - A.T).data)) == 0.0
+    assert cr.transpose(R).src_areas is R.dst_areas
