@@ -1,0 +1,277 @@
+"""Benchmark: Regridder build pairs/s and batched regrid! GB/s on one MI355X.
+
+Workload (the configuration the headline metric is quoted on):
+  source      0.25 degree lon-lat, 1440 x 720 cells (1441 x 721 vertices, poles included)
+  destination HEALPix nside 256, nested, 786 432 cells as 16-gons (4 points per side)
+  A           built in FP64 on the device
+  fields      256 FP32 fields laid out [n, 64, 4] cells fastest (what regrid! gets with
+              dims=1): seeded spherical-harmonic fields of degree <= 64 + 1 % uniform noise
+
+A step is one batched regrid! of all 256 fields (one kernel launch).  The
+plain-FP64 comparison is a loop of 256 single-field FP64 regrid! calls on the
+same GPU.  The build is timed from device-resident grids to device CSR(A),
+CSR(A^T) and areas.  The CPU baseline is the FP64 C restatement in oracle/,
+timed on this host.
+
+  python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--skip-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "Regridder build pairs/s; batched regrid! GB/s vs plain FP64, same GPU"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def legendre_table(lmax: int, x: np.ndarray) -> np.ndarray:
+    """4pi-normalised associated Legendre functions P[j, l, m] at x = sin(lat)."""
+    u = np.sqrt(np.clip(1.0 - x * x, 0.0, None))
+    P = np.zeros((x.shape[0], lmax + 1, lmax + 1))
+    P[:, 0, 0] = 1.0
+    for m in range(1, lmax + 1):
+        f = math.sqrt(3.0) if m == 1 else math.sqrt((2 * m + 1) / (2 * m))
+        P[:, m, m] = f * u * P[:, m - 1, m - 1]
+    for m in range(0, lmax):
+        P[:, m + 1, m] = math.sqrt(2 * m + 3) * x * P[:, m, m]
+    for m in range(0, lmax + 1):
+        for l in range(m + 2, lmax + 1):
+            a = math.sqrt((4 * l * l - 1) / (l * l - m * m))
+            b = math.sqrt(((l - 1) ** 2 - m * m) / (4 * (l - 1) ** 2 - 1))
+            P[:, l, m] = a * (x * P[:, l - 1, m] - b * P[:, l - 2, m])
+    return P
+
+
+def synth_lonlat_fields(torch, nlon, nlat, nfields, lmax, seed, noise=0.01):
+    """(nfields, nlat*nlon) FP32 on the device: sum_{l<=lmax, m} c_lm Pbar_lm(sin lat)
+    (cos m lon, sin m lon) at cell centres, c ~ N(0,1)/sqrt(#terms), + noise*U[0,1)."""
+    lat = np.radians(-90.0 + (np.arange(nlat) + 0.5) * 180.0 / nlat)
+    lon = np.radians((np.arange(nlon) + 0.5) * 360.0 / nlon)
+    P = torch.tensor(legendre_table(lmax, np.sin(lat)), device="cuda")  # (nlat, L, M)
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    nterms = (lmax + 1) * (lmax + 2)
+    ca = torch.randn(nfields, lmax + 1, lmax + 1, generator=g, dtype=torch.float64).tril().cuda()
+    cb = torch.randn(nfields, lmax + 1, lmax + 1, generator=g, dtype=torch.float64).tril().cuda()
+    ca, cb = ca / math.sqrt(nterms), cb / math.sqrt(nterms)
+    m = torch.arange(lmax + 1, device="cuda", dtype=torch.float64)
+    lon_t = torch.tensor(lon, device="cuda")
+    cosm, sinm = torch.cos(m[:, None] * lon_t[None]), torch.sin(m[:, None] * lon_t[None])  # (M, nlon)
+    out = torch.empty(nfields, nlat * nlon, dtype=torch.float32, device="cuda")
+    u = torch.rand(nfields, nlat * nlon, generator=g, dtype=torch.float32).cuda()
+    for f0 in range(0, nfields, 32):
+        f1 = min(nfields, f0 + 32)
+        A = torch.einsum("jlm,flm->fjm", P, ca[f0:f1])
+        B = torch.einsum("jlm,flm->fjm", P, cb[f0:f1])
+        F = torch.matmul(A, cosm) + torch.matmul(B, sinm)  # (f, nlat, nlon)
+        out[f0:f1] = (1.0 + F.reshape(f1 - f0, -1)).float() + noise * u[f0:f1]
+    return out
+
+
+def apply_bytes(nnz, n_rows, n_cols, k, elem):
+    """Algorithmic HBM bytes of one apply: int32 col + FP64 val per non-zero, int64
+    row pointers, one FP64 area (reciprocal) per row, k fields read and written."""
+    return nnz * 12 + (n_rows + 1) * 8 + n_rows * 8 + k * (n_cols + n_rows) * elem
+
+
+def read_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 PMC summary, if present."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline leg")
+    ap.add_argument("--builds", type=int, default=3, help="timed device builds")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the apply would shard "
+              "by fields with no communication, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import __graft_entry__ as ge
+    cr = ge.load_package()
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.cuda.get_device_name(0)
+    log(f"[bench] device {dev}; library {cr.version()}")
+
+    NLON, NLAT, NSIDE, STEP, K = 1440, 720, 256, 4, 256
+    t0 = time.time()
+    src_pts = cr.lonlat_points(NLON, NLAT)
+    src = cr.Grid.from_points(src_pts)
+    dst = cr.Grid.healpix(NSIDE, "nested", step=STEP)
+    torch.cuda.synchronize()
+    log(f"[bench] grids ready ({time.time() - t0:.1f} s): src {src.ncells} cells, dst {dst.ncells} cells")
+
+    # ---- build ---------------------------------------------------------------
+    R = cr.Regridder(dst, src)  # warm-up build (also the regridder used below)
+    times = []
+    for _ in range(args.builds):
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        Rb = cr.Regridder(dst, src)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - tb)
+        del Rb
+    st = R.stats()
+    build_s = float(np.median(times))
+    nnz, ncand = R.nnz, st["n_candidates"]
+    log(f"[bench] build: nnz={nnz} candidates={ncand} wall={build_s * 1e3:.1f} ms "
+        f"(stages ms: bounds {st['ms_bounds']:.2f} tree {st['ms_tree']:.2f} search {st['ms_search']:.2f} "
+        f"clip {st['ms_clip']:.2f} assemble {st['ms_assemble']:.2f})")
+
+    # ---- fields --------------------------------------------------------------
+    n_dst, n_src = R.shape
+    X2 = synth_lonlat_fields(torch, NLON, NLAT, K, 64, seed=0)  # (256, n_src) = Julia [n, 64, 4]
+    X = X2.view(4, 64, n_src).permute(2, 1, 0)  # (n_src, 64, 4), strides (1, n, 64n)
+    Y2 = torch.zeros(K, n_dst, dtype=torch.float32, device="cuda")
+    Y = Y2.view(4, 64, n_dst).permute(2, 1, 0)
+    torch.cuda.synchronize()
+
+    # ---- batched FP32 apply (the step) ----------------------------------------
+    variant, tune_ms = cr.regrid_tune(Y, R, X, dims=1)  # picks the kernel variant once
+    log(f"[bench] autotuned cells-fastest variant {variant} ({tune_ms:.4f} ms)")
+    launch = cr.regrid_plan(Y, R, X, dims=1)
+    for _ in range(args.warmup):
+        launch()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    tw = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        launch()
+    e1.record()
+    torch.cuda.synchronize()
+    wall_steps = time.perf_counter() - tw
+    ms_step = e0.elapsed_time(e1) / args.steps
+    B32 = apply_bytes(nnz, n_dst, n_src, K, 4)
+    gbps = B32 / (ms_step * 1e-3) / 1e9
+    log(f"[bench] batched FP32 apply: {ms_step:.4f} ms/step, {gbps:.0f} GB/s algorithmic "
+        f"({B32 / 1e9:.3f} GB/launch), wall {wall_steps:.3f} s")
+
+    # ---- loop of 256 plain FP64 single-field applies --------------------------
+    X64 = X2.double()
+    Y64 = torch.zeros(K, n_dst, dtype=torch.float64, device="cuda")
+    loop = [cr.regrid_plan(Y64[f], R, X64[f]) for f in range(K)]
+    for fn in loop:
+        fn()
+    torch.cuda.synchronize()
+    reps = 3
+    e0.record()
+    for _ in range(reps):
+        for fn in loop:
+            fn()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_loop = e0.elapsed_time(e1) / reps
+    B64 = apply_bytes(nnz, n_dst, n_src, 1, 8)
+    gbps_loop = K * B64 / (ms_loop * 1e-3) / 1e9
+    log(f"[bench] FP64 loop of {K} single-field applies: {ms_loop:.3f} ms, {gbps_loop:.0f} GB/s; "
+        f"batched speed-up {ms_loop / ms_step:.2f}x")
+
+    # ---- accuracy vs FP64 -----------------------------------------------------
+    Yd = Y2.double()
+    scale = Y64.abs().amax(dim=1)
+    rel_err = float(((Yd - Y64).abs().amax(dim=1) / scale).max())
+    a_d = torch.tensor(R.dst_areas, device="cuda")
+    a_s = torch.tensor(R.src_areas, device="cuda")
+    tot_s = (X64 * a_s).sum(1)
+    drift = float(((Yd * a_d).sum(1) - tot_s).abs().div((X64 * a_s).abs().sum(1)).max())
+    drift64 = float(((Y64 * a_d).sum(1) - tot_s).abs().div((X64 * a_s).abs().sum(1)).max())
+    log(f"[bench] FP32 vs FP64: max rel field error {rel_err:.2e}; conservation drift FP32 {drift:.2e}, "
+        f"FP64 {drift64:.2e}")
+
+    # ---- CPU baseline (oracle, bounded sample) --------------------------------
+    cpu = None
+    build_speedup = None
+    if not args.skip_cpu:
+        orc = ge.load_oracle()
+        nth = min(16, os.cpu_count() or 1)
+        dcells = dst.cells()
+        scells = cr.points_to_cells(src_pts)
+        tc = time.perf_counter()
+        ref = orc.build(dcells, scells, clip_is_dst=False, nthreads=nth)
+        cpu_build_s = time.perf_counter() - tc
+        cpu_pairs = ref.nnz / cpu_build_s
+        build_speedup = (nnz / build_s) / cpu_pairs
+        # FP64 single-field applies, 1 thread (mul! is single-threaded): 16 of the 256
+        rp, col, val = R.csr_arrays()
+        xs = X64[:16].cpu().numpy()
+        tc = time.perf_counter()
+        for f in range(16):
+            orc.spmv(rp, col, val, xs[f], R.dst_areas, 1)
+        cpu_apply_s = (time.perf_counter() - tc) / 16
+        cpu_gbps = B64 / cpu_apply_s / 1e9
+        # parity of one field against the oracle apply
+        y0 = orc.spmv(rp, col, val, xs[0], R.dst_areas, nth)
+        fp64_err = float(np.max(np.abs(Y64[0].cpu().numpy() - y0)) / np.max(np.abs(y0)))
+        cpu = {"value": round(cpu_gbps, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+               "sample": "16 of the 256 FP64 single-field applies on oracle/ (1 thread, as mul!)",
+               "build_pairs_per_s": round(cpu_pairs, 1), "build_threads": nth,
+               "build_sample": "full build of this workload on oracle/",
+               "build_seconds": round(cpu_build_s, 3), "cpu_nnz": ref.nnz,
+               "fp64_field_rel_err_vs_oracle": fp64_err}
+        log(f"[bench] CPU oracle: build {cpu_build_s:.2f} s ({cpu_pairs / 1e6:.2f} M pairs/s, {nth} threads), "
+            f"apply {cpu_apply_s * 1e3:.1f} ms/field ({cpu_gbps:.2f} GB/s, 1 thread); "
+            f"GPU build speed-up {build_speedup:.1f}x; FP64 GPU vs oracle {fp64_err:.1e}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(gbps, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 fields, f64 A",
+        "data": "synthetic",
+        "config": {
+            "workload": "0.25deg lon-lat 1440x720 -> HEALPix nside 256 nested (16-gon cells); "
+                        "256 FP32 fields [n,64,4] cells fastest, dims=1",
+            "fields": K, "n_src": n_src, "n_dst": n_dst, "nnz": nnz, "apply_variant": variant,
+        },
+        "build": {"pairs_per_s": round(nnz / build_s, 1), "ms": round(build_s * 1e3, 3), "nnz": nnz,
+                  "candidates": ncand, "candidate_pairs_per_s": round(ncand / build_s, 1),
+                  "stage_ms": {k: round(st[k], 3) for k in
+                               ("ms_bounds", "ms_tree", "ms_search", "ms_clip", "ms_assemble", "ms_total")},
+                  "speedup_vs_cpu": None if build_speedup is None else round(build_speedup, 2)},
+        "fp64_loop": {"ms": round(ms_loop, 4), "GBps": round(gbps_loop, 2),
+                      "batched_speedup": round(ms_loop / ms_step, 3)},
+        "accuracy": {"fp32_max_rel_field_error": rel_err, "fp32_conservation_drift": drift,
+                     "fp64_conservation_drift": drift64},
+        "roofline": {"bound": "hbm", "achieved": round(gbps, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(gbps / HBM_PEAK_GBPS, 4),
+                     "traffic": read_traffic("spmm_f32_cells_c3_pmc.txt")},
+        "cpu_baseline": cpu,
+        "device": dev,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

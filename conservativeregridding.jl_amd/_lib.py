@@ -77,6 +77,8 @@ SIGNATURES = {
     "cr_regridder_destroy": (None, [c_vp]),
     "cr_regrid": (c_i32, [c_vp, c_i32, c_i32, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, c_i32,
                           c_vp]),
+    "cr_regrid_tune": (c_i32, [c_vp, c_i32, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, _i32p,
+                               ctypes.POINTER(ctypes.c_float), c_vp]),
 }
 
 

@@ -24,64 +24,26 @@
 
 #include "regridder.hpp"
 #include "sphgeom.hpp"
+#include "tree.hpp"
 
 namespace cr {
 
-constexpr int kTreeFanout = 8;
-constexpr int kMaxLevels = 24;
-constexpr float kCapAll = 4.0f;  // radius that overlaps everything (chord <= 2)
-
-struct Levels {
-    int64_t off[kMaxLevels];
-    int64_t size[kMaxLevels];
-    int top;
-};
-
-// Area (unit sphere) and bounding cap of each cell.  The cap is a ball
-// (centre, chord radius) around the normalised vertex mean; for cells inside a
-// hemisphere it holds the whole cell.  FP32 copies are padded outward.
+// ---- kernels over tree.hpp -------------------------------------------------
+// Area (unit sphere) and bounding cap of each cell.
 __global__ void k_cell_bounds(const double *__restrict__ cells, int64_t n, int nv,
                               double *__restrict__ area, float4 *__restrict__ cap) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     const double *p = cells + 3 * nv * c;
     area[c] = ring_area(p, nv);
-    d3 s = mk3(0, 0, 0);
-    for (int k = 0; k < nv; ++k) s = s + load3(p + 3 * k);
-    double L = sqrt(dot(s, s));
-    if (L < 1e-12) {
-        cap[c] = make_float4(0.f, 0.f, 1.f, kCapAll);
-        return;
-    }
-    d3 m = (1.0 / L) * s;
-    double r = 0.0;
-    for (int k = 0; k < nv; ++k) {
-        d3 d = load3(p + 3 * k) - m;
-        r = fmax(r, sqrt(dot(d, d)));
-    }
-    float rf = r > 1.4 ? kCapAll : (float)(r * (1.0 + 1e-6) + 2e-6);
-    cap[c] = make_float4((float)m.x, (float)m.y, (float)m.z, rf);
-}
-
-__device__ __forceinline__ uint32_t spread3(uint32_t v) {
-    v &= 0x3ffu;
-    v = (v | (v << 16)) & 0x030000ffu;
-    v = (v | (v << 8)) & 0x0300f00fu;
-    v = (v | (v << 4)) & 0x030c30c3u;
-    v = (v | (v << 2)) & 0x09249249u;
-    return v;
+    cap[c] = cell_cap(p, nv);
 }
 
 __global__ void k_morton(const float4 *__restrict__ cap, int64_t n, uint32_t *__restrict__ key,
                          int32_t *__restrict__ idx) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
-    float4 q = cap[c];
-    auto quant = [](float x) {
-        int v = (int)((x + 1.0f) * 512.0f);
-        return (uint32_t)min(1023, max(0, v));
-    };
-    key[c] = spread3(quant(q.x)) | (spread3(quant(q.y)) << 1) | (spread3(quant(q.z)) << 2);
+    key[c] = morton_key(cap[c]);
     idx[c] = (int32_t)c;
 }
 
@@ -91,45 +53,17 @@ __global__ void k_gather_caps(const float4 *__restrict__ cap, const int32_t *__r
     if (i < n) leaf[i] = cap[perm[i]];
 }
 
-// Parent cap = ball around the normalised mean of the child centres that holds
-// every child ball (triangle inequality on chords).
 __global__ void k_build_level(const float4 *__restrict__ child, int64_t nchild,
                               float4 *__restrict__ parent, int64_t nparent) {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nparent) return;
     int64_t c0 = p * kTreeFanout, c1 = min(nchild, c0 + kTreeFanout);
-    d3 s = mk3(0, 0, 0);
-    bool all = false;
-    for (int64_t c = c0; c < c1; ++c) {
-        float4 q = child[c];
-        s = s + mk3(q.x, q.y, q.z);
-        all |= q.w >= 2.0f;
-    }
-    double L = sqrt(dot(s, s));
-    if (all || L < 1e-9) {
-        parent[p] = make_float4(0.f, 0.f, 1.f, kCapAll);
-        return;
-    }
-    d3 m = (1.0 / L) * s;
-    double r = 0.0;
-    for (int64_t c = c0; c < c1; ++c) {
-        float4 q = child[c];
-        d3 d = mk3(q.x, q.y, q.z) - m;
-        r = fmax(r, sqrt(dot(d, d)) + (double)q.w);
-    }
-    float rf = r >= 2.0 ? kCapAll : (float)(r * (1.0 + 1e-6) + 2e-6);
-    parent[p] = make_float4((float)m.x, (float)m.y, (float)m.z, rf);
+    parent[p] = parent_cap(child, c0, c1);
 }
 
-__device__ __forceinline__ bool caps_overlap(const float4 &a, const float4 &b) {
-    float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-    float rr = a.w + b.w;
-    return dx * dx + dy * dy + dz * dz <= rr * rr;
-}
-
-// One thread per destination cell; stackless depth-first walk of the implicit
-// kTreeFanout-ary tree (level 0 = Morton-sorted source cells).  FILL=false
-// counts candidates, FILL=true writes them at the scanned offsets.
+// One thread per destination cell walks the source tree.  FILL=false counts
+// candidates; FILL=true writes them at the scanned offsets (never past the
+// row's segment) in the same deterministic order.
 template <bool FILL>
 __global__ void __launch_bounds__(256) k_traverse(const float4 *__restrict__ qcap, int64_t nq,
                                                   const float4 *__restrict__ nodes, Levels lv,
@@ -139,45 +73,21 @@ __global__ void __launch_bounds__(256) k_traverse(const float4 *__restrict__ qca
                                                   int32_t *__restrict__ cand_dst) {
     int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= nq) return;
-    float4 q = qcap[d];
-    int64_t pos = FILL ? count_or_offset[d] : 0;
-    const int64_t end = FILL ? count_or_offset[d + 1] : 0;  // never write past the row
-    int64_t cnt = 0;
-    int L = lv.top;
-    int64_t i = 0;
-    for (;;) {
-        bool hit = caps_overlap(q, nodes[lv.off[L] + i]);
-        if (hit) {
-            if (L == 0) {
-                if (FILL) {
-                    if (pos < end) {
-                        cand_src[pos] = perm[i];
-                        cand_dst[pos] = (int32_t)d;
-                        ++pos;
-                    }
-                } else {
-                    ++cnt;
-                }
-            } else {
-                --L;
-                i *= kTreeFanout;
-                continue;
+    if (FILL) {
+        int64_t pos = count_or_offset[d];
+        const int64_t end = count_or_offset[d + 1];
+        tree_query(qcap[d], nodes, lv, [&](int64_t i) {
+            if (pos < end) {
+                cand_src[pos] = perm[i];
+                cand_dst[pos] = (int32_t)d;
+                ++pos;
             }
-        }
-        // next sibling, or climb until one exists
-        for (;;) {
-            if (L == lv.top) goto done;
-            int64_t nxt = i + 1;
-            if ((nxt % kTreeFanout) != 0 && nxt < lv.size[L]) {
-                i = nxt;
-                break;
-            }
-            i /= kTreeFanout;
-            ++L;
-        }
+        });
+    } else {
+        int64_t cnt = 0;
+        tree_query(qcap[d], nodes, lv, [&](int64_t) { ++cnt; });
+        count_or_offset[d] = cnt;
     }
-done:
-    if (!FILL) count_or_offset[d] = cnt;
 }
 
 // One thread per candidate: FP64 area of (subject ∩ clip) via the streaming
@@ -195,8 +105,9 @@ __global__ void __launch_bounds__(256) k_clip(const int32_t *__restrict__ cand_d
     int64_t d = cand_dst[c], s = cand_src[c];
     const double *dp = dst_cells + 3 * (int64_t)dst_nv * d;
     const double *sp = src_cells + 3 * (int64_t)src_nv * s;
-    double a = clip_is_dst ? clip_area_stream<NC>(dp, dst_nv, sp, src_nv)
-                           : clip_area_stream<NC>(sp, src_nv, dp, dst_nv);
+    const double *clip = clip_is_dst ? dp : sp, *subj = clip_is_dst ? sp : dp;
+    const int nc = clip_is_dst ? dst_nv : src_nv, ns = clip_is_dst ? src_nv : dst_nv;
+    double a = clip_area_stream<NC>(clip, nc, subj, ns);
     double amin = fmin(a_src[s], a_dst[d]);
     val[c] = a > thr * amin ? a : 0.0;
 }
@@ -416,21 +327,7 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     tm.mark();  // 1
 
     // 2. tree over source caps
-    Levels lv{};
-    {
-        int64_t n = ns, off = 0;
-        int L = 0;
-        for (;;) {
-            CR_REQUIRE(L < kMaxLevels, CR_INTERNAL, "tree too deep");
-            lv.off[L] = off;
-            lv.size[L] = n;
-            off += n;
-            if (n == 1) break;
-            n = (n + kTreeFanout - 1) / kTreeFanout;
-            ++L;
-        }
-        lv.top = L;
-    }
+    Levels lv = make_levels(ns);
     int64_t nnodes = lv.off[lv.top] + 1;
     DevBuf<float4> nodes(nnodes);
     DevBuf<int32_t> perm(ns);

@@ -30,6 +30,7 @@ struct Csr {
     DevBuf<double> val;
     DevBuf<double> inv_area;  // 1 / area of the row cells (FP32 batched path)
     ApplyPlan plan;
+    int tuned_cells_variant = -1;  // set by cr_regrid_tune; -1: default
 };
 
 }  // namespace cr

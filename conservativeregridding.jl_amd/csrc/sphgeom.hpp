@@ -108,26 +108,22 @@ CR_HD d3 crossing(const d3 &p, double sp, const d3 &q, double sq) {
 template <int K, int NC>
 CR_HD void pipe_push(ClipPipe<NC> &P, const d3 &v);
 
-// Sutherland-Hodgman rule for edge p->q at stage K: emits 0, 1 or 2 vertices.
+// Sutherland-Hodgman rule for edge p->q at stage K: a crossing when the side
+// changes, then q when it is inside.  The (at most two) outputs are forwarded
+// by a loop that is not unrolled, so each stage's downstream code exists once
+// and the kernel grows linearly, not exponentially, with the number of stages.
 template <int K, int NC>
 CR_HD void pipe_edge(ClipPipe<NC> &P, const d3 &p, double sp, const d3 &q, double sq) {
-    d3 o1, o2;
-    int m = 0;
-    if (sq >= 0.0) {
-        if (sp < 0.0) {
-            o1 = crossing(p, sp, q, sq);
-            o2 = q;
-            m = 2;
-        } else {
-            o1 = q;
-            m = 1;
-        }
-    } else if (sp >= 0.0) {
-        o1 = crossing(p, sp, q, sq);
-        m = 1;
+    const bool qin = sq >= 0.0;
+    const bool crosses = (sp >= 0.0) != qin;
+    d3 x = q;
+    if (crosses) x = crossing(p, sp, q, sq);
+    const int m = (int)crosses + (int)qin;
+#pragma unroll 1
+    for (int e = 0; e < m; ++e) {
+        d3 u = (e == 0) ? x : q;
+        pipe_push<K + 1, NC>(P, u);
     }
-    if (m >= 1) pipe_push<K + 1, NC>(P, o1);
-    if (m >= 2) pipe_push<K + 1, NC>(P, o2);
 }
 
 template <int K, int NC>

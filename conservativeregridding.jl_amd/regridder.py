@@ -372,3 +372,51 @@ def regrid(R: Regridder, src_field, **kw):
     shape[0 if a.ndim == 1 else dims - 1] = n_out
     out = np.zeros(shape, dtype=a.dtype)
     return regrid_(out, R, a, **kw)
+
+
+def regrid_plan(dst_field, R: Regridder, src_field, *, dims: int = 1, normalize: bool = True):
+    """Validate once and return a zero-argument callable that re-launches
+    ``regrid_(dst_field, R, src_field)`` on the current stream with no host work
+    beyond the launch (device fields only).  Suitable for timing loops and for
+    capture into a HIP graph (the launch allocates nothing)."""
+    import torch
+    if not (isinstance(dst_field, torch.Tensor) and dst_field.is_cuda and isinstance(src_field, torch.Tensor)
+            and src_field.is_cuda):
+        raise MethodError("regrid_plan needs device (CUDA) tensors")
+    regrid_(dst_field, R, src_field, dims=dims, normalize=normalize)  # validates + first launch
+    ns = src_field.ndim
+    dim = 0 if ns == 1 else dims - 1
+    nf, sf, df, outer = _merge_fields(tuple(src_field.shape), src_field.stride(), dst_field.stride(), dim)
+    if outer:
+        raise MethodError("regrid_plan needs field axes that flatten into one stride")
+    code = _dtype_code(src_field.dtype)
+    args = (R._core.h, 1 if R._t else 0, code, nf, src_field.data_ptr(), src_field.stride(dim), sf,
+            dst_field.data_ptr(), dst_field.stride(dim), df, 1 if normalize else 0)
+    fn = L.lib.cr_regrid
+    keep = (R, dst_field, src_field)
+
+    def launch():
+        st = fn(*args, torch.cuda.current_stream().cuda_stream)
+        if st:
+            L.check(st, "regrid!")
+        return keep[1]
+
+    return launch
+
+
+def regrid_tune(dst_field, R: Regridder, src_field, *, dims: int = 1):
+    """Autotune the FP32 cells-fastest batched apply for this direction of ``R``
+    on these device fields (``dst_field`` receives the regridded result).  The
+    fastest kernel variant is kept in the regridder for later calls.  Returns
+    (variant index, milliseconds per launch)."""
+    import torch
+    regrid_(dst_field, R, src_field, dims=dims)  # validation
+    dim = 0 if src_field.ndim == 1 else dims - 1
+    nf, sf, df, outer = _merge_fields(tuple(src_field.shape), src_field.stride(), dst_field.stride(), dim)
+    if outer or _dtype_code(src_field.dtype) != L.CR_F32:
+        raise MethodError("regrid_tune needs FP32 fields whose field axes flatten into one stride")
+    chosen, ms = ctypes.c_int32(-1), ctypes.c_float(0.0)
+    L.check(L.lib.cr_regrid_tune(R._core.h, 1 if R._t else 0, nf, src_field.data_ptr(), src_field.stride(dim), sf,
+                                 dst_field.data_ptr(), dst_field.stride(dim), df, ctypes.byref(chosen),
+                                 ctypes.byref(ms), torch.cuda.current_stream().cuda_stream), "regrid_tune")
+    return chosen.value, ms.value

@@ -166,6 +166,18 @@ cr_status cr_regrid(const cr_regridder *R, int32_t transpose, int32_t dtype, int
                     void *dst, int64_t dst_cell_stride, int64_t dst_field_stride,
                     int32_t normalize, void *stream);
 
+/*
+ * Autotuner for the FP32 cells-fastest batched apply (no reference
+ * counterpart: the reference's slice loop has no tunable).  Times every kernel
+ * variant on the given device fields (dst is overwritten with the regridded
+ * result) and stores the fastest in the handle for that direction; later
+ * cr_regrid calls of that direction use it.  Synchronises the stream.
+ */
+cr_status cr_regrid_tune(cr_regridder *R, int32_t transpose, int64_t nfields, const void *src,
+                         int64_t src_cell_stride, int64_t src_field_stride, void *dst,
+                         int64_t dst_cell_stride, int64_t dst_field_stride, int32_t *chosen,
+                         float *best_ms, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

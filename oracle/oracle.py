@@ -73,6 +73,11 @@ def lib():
     return L
 
 
+def default_threads() -> int:
+    """Worker threads: the host's cores, capped at 16 (one GPU's share of a box)."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
 def _d(a):
     return a.ctypes.data_as(_dp)
 
@@ -106,7 +111,7 @@ def cell_areas(cells, nthreads: int = 0) -> np.ndarray:
     """Areas of a polygon table, shape (ncells, nv, 3)."""
     c = _f64(cells)
     out = np.empty(c.shape[0])
-    lib().orc_cell_areas(_d(c), c.shape[0], c.shape[1], _d(out), nthreads or os.cpu_count() or 1)
+    lib().orc_cell_areas(_d(c), c.shape[0], c.shape[1], _d(out), nthreads or default_threads())
     return out
 
 
@@ -150,7 +155,7 @@ def build(dst_cells, src_cells, clip_is_dst: bool = True, threshold_rel: float =
     """Regridder build on the unit sphere.  ``dst_cells``/``src_cells`` are
     polygon tables (ncells, nv, 3).  ``clip_is_dst`` picks the convex side."""
     d, s = _f64(dst_cells), _f64(src_cells)
-    nth = nthreads or (os.cpu_count() or 1)
+    nth = nthreads or default_threads()
     r0, r1 = (0, d.shape[0]) if row_range is None else row_range
     L = lib()
     h = L.orc_build(_d(d), d.shape[0], d.shape[1], _d(s), s.shape[0], s.shape[1],

@@ -194,3 +194,25 @@ def test_normalize_in_place_keeps_results(cr):
     assert R.intersections.max() == 1.0
     d2 = cr.regrid(R, s)
     torch.testing.assert_close(d1, d2, rtol=1e-14, atol=0)
+
+
+def test_autotuned_variant_is_accurate(cr, c2):
+    _case("tune")
+    R, A = c2
+    n_dst, n_src = A.shape
+    k = 64
+    x64 = _fields(n_src, k, 9)
+    x = torch.tensor(x64.T.copy(), dtype=torch.float32, device=DEV).T  # cells fastest (n, k)
+    y = torch.zeros(k, n_dst, dtype=torch.float32, device=DEV).T
+    variant, ms = cr.regrid_tune(y, R, x)
+    assert variant >= 0 and ms > 0
+    y.zero_()
+    cr.regrid_(y, R, x)
+    ref = (A @ x.cpu().numpy().astype(np.float64)) / R.dst_areas[:, None]
+    assert np.max(np.abs(y.cpu().numpy() - ref)) / np.max(np.abs(ref)) <= 1e-6
+    # the transpose direction keeps its own (untuned) choice and stays correct
+    back = torch.zeros(k, n_src, dtype=torch.float32, device=DEV).T
+    cr.regrid_(back, cr.transpose(R), y)
+    trp = A.T.tocsr()
+    ref_b = (trp @ y.cpu().numpy().astype(np.float64)) / R.src_areas[:, None]
+    assert np.max(np.abs(back.cpu().numpy() - ref_b)) / np.max(np.abs(ref_b)) <= 1e-6
