@@ -178,4 +178,205 @@ CR_HD double ring_area(const double *xyz, int n) {
     return a.area();
 }
 
+
+// ============================================================================
+// Backward: d(area of subject ∩ clip) / d(vertices), by provenance.
+//
+// Every vertex of the clipped polygon is, in exact arithmetic, a subject
+// vertex, a clip vertex, or the crossing of two great circles (a subject edge
+// with a clip edge, or two clip edges).  The pipeline below is the forward one
+// with a label carried next to each vertex; a crossing's label comes from the
+// line the clipped segment lies on (the line its two endpoints share).  The
+// area stage differentiates each fan triangle,
+//   E = 2 atan2(T, D),  T = a.(b x c),  D = 1 + a.b + b.c + c.a,
+//   dE/da = 2 (D (b x c) - T (b + c)) / (T^2 + D^2)   (and cyclic),
+// and hands (label, position, gradient) to a sink, which chains through the
+// crossing x = y/|y|, y = +-(n1 x n2), n = p x q, to the input vertices.
+// ============================================================================
+
+// label: bits 30-31 type, low bits indices
+constexpr int32_t kLabS = 0, kLabC = 1, kLabXF = 2, kLabXE = 3;
+CR_HD int32_t lab_make(int32_t type, int32_t a, int32_t b) { return (type << 30) | (a << 8) | b; }
+CR_HD int32_t lab_type(int32_t l) { return (l >> 30) & 3; }
+CR_HD int32_t lab_hi(int32_t l) { return (l >> 8) & 0x3fffff; }
+CR_HD int32_t lab_lo(int32_t l) { return l & 0xff; }
+
+// line ids: subject edge e -> e, clip edge k -> kClipLine | k
+constexpr int32_t kClipLine = 1 << 29;
+
+template <int NC>
+CR_HD void lab_lines(int32_t l, int ns, int32_t &l1, int32_t &l2) {
+    int32_t t = lab_type(l), a = lab_hi(l), b = lab_lo(l);
+    if (t == kLabS) { l1 = (a + ns - 1) % ns; l2 = a; }
+    else if (t == kLabC) { l1 = kClipLine | ((b + NC - 1) % NC); l2 = kClipLine | b; }
+    else if (t == kLabXF) { l1 = a; l2 = kClipLine | b; }
+    else { l1 = kClipLine | a; l2 = kClipLine | b; }
+}
+
+// Label of the crossing of segment p->q with clip edge K.
+template <int NC>
+CR_HD int32_t lab_crossing(int32_t lp, int32_t lq, int K, int ns) {
+    int32_t p1, p2, q1, q2;
+    lab_lines<NC>(lp, ns, p1, p2);
+    lab_lines<NC>(lq, ns, q1, q2);
+    int32_t line = (p1 == q1 || p1 == q2) ? p1 : ((p2 == q1 || p2 == q2) ? p2 : -1);
+    if (line < 0) return -1;  // no shared line (degenerate input): no gradient
+    if (!(line & kClipLine)) return lab_make(kLabXF, line, K);
+    int a = line & 0xff;
+    if (a == (K + NC - 1) % NC) return lab_make(kLabC, 0, K);
+    if (a == (K + 1) % NC) return lab_make(kLabC, 0, (K + 1) % NC);
+    return lab_make(kLabXE, a, K);
+}
+
+CR_HD d3 tangential(const d3 &g, const d3 &x) { return g - dot(g, x) * x; }
+
+template <int NC, class Sink>
+struct GradPipe {
+    d3 anchor[NC], nrm[NC];
+    d3 first[NC], prev[NC];
+    double sfirst[NC], sprev[NC];
+    int32_t lfirst[NC], lprev[NC];
+    bool started[NC];
+    int ns;
+    double coef;
+    Sink *sink;
+    // area stage
+    d3 w0, wp, g0;
+    int32_t l0, lp;
+    int n;
+
+    CR_HD void area_push(const d3 &w, int32_t l) {
+        if (n == 0) {
+            w0 = w;
+            l0 = l;
+            g0 = mk3(0, 0, 0);
+        } else if (n >= 2) {
+            d3 a = w0, b = wp, c = w;
+            double T = dot(a, cross(b, c));
+            double D = 1.0 + dot(a, b) + dot(b, c) + dot(c, a);
+            double f = 2.0 / (T * T + D * D);
+            g0 = g0 + f * (D * cross(b, c) - T * (b + c));
+            (*sink)(lp, b, (coef * f) * (D * cross(c, a) - T * (a + c)));
+            (*sink)(l, c, (coef * f) * (D * cross(a, b) - T * (a + b)));
+        }
+        wp = w;
+        lp = l;
+        ++n;
+    }
+};
+
+template <int K, int NC, class Sink>
+CR_HD void gpipe_push(GradPipe<NC, Sink> &P, const d3 &v, int32_t l);
+
+template <int K, int NC, class Sink>
+CR_HD void gpipe_edge(GradPipe<NC, Sink> &P, const d3 &p, double sp, int32_t lp, const d3 &q,
+                      double sq, int32_t lq) {
+    const bool qin = sq >= 0.0;
+    const bool crosses = (sp >= 0.0) != qin;
+    d3 x = q;
+    int32_t lx = lq;
+    if (crosses) {
+        x = crossing(p, sp, q, sq);
+        lx = lab_crossing<NC>(lp, lq, K, P.ns);
+    }
+    const int m = (int)crosses + (int)qin;
+#pragma unroll 1
+    for (int e = 0; e < m; ++e) {
+        d3 u = (e == 0) ? x : q;
+        int32_t lu = (e == 0) ? lx : lq;
+        gpipe_push<K + 1, NC, Sink>(P, u, lu);
+    }
+}
+
+template <int K, int NC, class Sink>
+CR_HD void gpipe_push(GradPipe<NC, Sink> &P, const d3 &v, int32_t l) {
+    if constexpr (K == NC) {
+        P.area_push(v, l);
+    } else {
+        double s = dot(P.nrm[K], v - P.anchor[K]);
+        if (!P.started[K]) {
+            P.started[K] = true;
+            P.first[K] = v;
+            P.sfirst[K] = s;
+            P.lfirst[K] = l;
+        } else {
+            gpipe_edge<K, NC, Sink>(P, P.prev[K], P.sprev[K], P.lprev[K], v, s, l);
+        }
+        P.prev[K] = v;
+        P.sprev[K] = s;
+        P.lprev[K] = l;
+    }
+}
+
+template <int K, int NC, class Sink>
+CR_HD void gpipe_close(GradPipe<NC, Sink> &P) {
+    if constexpr (K < NC) {
+        if (P.started[K])
+            gpipe_edge<K, NC, Sink>(P, P.prev[K], P.sprev[K], P.lprev[K], P.first[K], P.sfirst[K],
+                                    P.lfirst[K]);
+        gpipe_close<K + 1, NC, Sink>(P);
+    }
+}
+
+// Scatter coef * d(area)/d(vertex) of (subject ∩ clip) through `sink`.
+template <int NC, class Sink>
+CR_HD void clip_area_backward(const double *clip, int nc, const double *subj, int ns, double coef,
+                              Sink &sink) {
+    GradPipe<NC, Sink> P;
+    P.ns = ns;
+    P.coef = coef;
+    P.sink = &sink;
+    P.n = 0;
+    d3 c[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) c[k] = load3(clip + 3 * (k < nc ? k : nc - 1));
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        d3 n = cross(c[k], c[(k + 1) % NC] - c[k]);
+        if (dot(n, n) < kDegenerateEdge2) n = mk3(0.0, 0.0, 0.0);
+        P.anchor[k] = c[k];
+        P.nrm[k] = n;
+        P.started[k] = false;
+    }
+    for (int i = 0; i < ns; ++i) gpipe_push<0, NC, Sink>(P, load3(subj + 3 * i), lab_make(kLabS, 0, 0) | i);
+    gpipe_close<0, NC, Sink>(P);
+    if (P.n >= 3) sink(P.l0, P.w0, coef * P.g0);
+}
+
+// Gradient w.r.t. the great-circle crossing x = y/|y| (y = +-(n1 x n2),
+// n1 = p1 x q1, n2 = p2 x q2) pushed back to p1, q1, p2, q2.
+CR_HD bool crossing_backward(const d3 &x, const d3 &g, const d3 &p1, const d3 &q1, const d3 &p2,
+                             const d3 &q2, d3 &gp1, d3 &gq1, d3 &gp2, d3 &gq2) {
+    d3 n1 = cross(p1, q1), n2 = cross(p2, q2);
+    d3 y = cross(n1, n2);
+    double L = sqrt(dot(y, y));
+    if (!(L > 1e-300)) return false;
+    double sigma = dot(y, x) >= 0.0 ? 1.0 : -1.0;
+    d3 yb = (sigma / L) * tangential(g, x);
+    d3 n1b = cross(n2, yb), n2b = cross(yb, n1);
+    gp1 = cross(q1, n1b);
+    gq1 = cross(n1b, p1);
+    gp2 = cross(q2, n2b);
+    gq2 = cross(n2b, p2);
+    return true;
+}
+
+// coef * d(signed ring area)/d(vertices), one call of emit(slot, grad) per vertex use.
+template <class Emit>
+CR_HD void ring_area_backward(const double *xyz, int n, double coef, Emit &&emit) {
+    if (n < 3) return;
+    d3 a = load3(xyz);
+    d3 ga = mk3(0, 0, 0);
+    for (int i = 1; i + 1 < n; ++i) {
+        d3 b = load3(xyz + 3 * i), c = load3(xyz + 3 * (i + 1));
+        double T = dot(a, cross(b, c));
+        double D = 1.0 + dot(a, b) + dot(b, c) + dot(c, a);
+        double f = coef * 2.0 / (T * T + D * D);
+        ga = ga + f * (D * cross(b, c) - T * (b + c));
+        emit(i, tangential(f * (D * cross(c, a) - T * (a + c)), b));
+        emit(i + 1, tangential(f * (D * cross(a, b) - T * (a + b)), c));
+    }
+    emit(0, tangential(ga, a));
+}
+
 }  // namespace cr

@@ -67,6 +67,10 @@ def lib():
     L.orc_result_free.argtypes = [ctypes.c_void_p]
     L.orc_spmv.restype = None
     L.orc_spmv.argtypes = [_i64p, _i32p, _dp, ctypes.c_int64, _dp, _dp, _dp, ctypes.c_int32]
+    L.orc_polygon_area_grad.restype = ctypes.c_double
+    L.orc_polygon_area_grad.argtypes = [_dp, ctypes.c_int32, _dp]
+    L.orc_clip_area_grad.restype = ctypes.c_double
+    L.orc_clip_area_grad.argtypes = [_dp, ctypes.c_int32, _dp, ctypes.c_int32, _dp, _dp]
     L.orc_transpose.restype = None
     L.orc_transpose.argtypes = [_i64p, _i32p, _dp, ctypes.c_int64, ctypes.c_int64, _i64p, _i32p, _dp]
     _lib = L
@@ -96,6 +100,22 @@ def clip_area(subject, clip) -> float:
     """Area of subject ∩ clip (clip convex, both CCW open rings of unit vectors)."""
     s, c = _f64(subject), _f64(clip)
     return lib().orc_clip_area(_d(s), s.shape[0], _d(c), c.shape[0])
+
+
+def polygon_area_grad(ring):
+    """(area, d area / d vertices) of area(normalize(ring)), forward-mode duals."""
+    r = _f64(ring)
+    g = np.empty_like(r)
+    a = lib().orc_polygon_area_grad(_d(r), r.shape[0], _d(g))
+    return a, g
+
+
+def clip_area_grad(subject, clip):
+    """(area, d/d subject, d/d clip) of area(normalize(subject) ∩ normalize(clip))."""
+    s, c = _f64(subject), _f64(clip)
+    gs, gc = np.empty_like(s), np.empty_like(c)
+    a = lib().orc_clip_area_grad(_d(s), s.shape[0], _d(c), c.shape[0], _d(gs), _d(gc))
+    return a, gs, gc
 
 
 def clip_polygon(subject, clip) -> np.ndarray:
