@@ -128,14 +128,16 @@ def main():
     # ---- build ---------------------------------------------------------------
     R = cr.Regridder(dst, src)  # warm-up build (also the regridder used below)
     times = []
+    st = None
     for _ in range(args.builds):
         torch.cuda.synchronize()
         tb = time.perf_counter()
         Rb = cr.Regridder(dst, src)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - tb)
+        st = Rb.stats()  # device stage times of a steady-state build
         del Rb
-    st = R.stats()
+    st = st or R.stats()
     build_s = float(np.median(times))
     nnz, ncand = R.nnz, st["n_candidates"]
     log(f"[bench] build: nnz={nnz} candidates={ncand} wall={build_s * 1e3:.1f} ms "

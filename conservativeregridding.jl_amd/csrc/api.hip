@@ -12,6 +12,19 @@ static thread_local std::string g_last_error;
 
 void set_error(const std::string &msg) { g_last_error = msg; }
 
+// Keep freed pool memory cached (default threshold 0 returns it at every sync).
+void configure_pool() {
+    static thread_local int done_for_device = -1;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev == done_for_device) return;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+        uint64_t thr = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+    done_for_device = dev;
+}
+
 // CSR(A) -> CSR(A^T) on the host (used by cr_regridder_from_csr only).
 static void host_transpose(int64_t nrows, int64_t ncols, const int64_t *rp, const int32_t *col,
                            const double *val, std::vector<int64_t> &trp, std::vector<int32_t> &tcol,

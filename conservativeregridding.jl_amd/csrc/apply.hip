@@ -3,147 +3,159 @@
 // Reference: perform_regridding! = mul!(dst, R.intersections, src) followed by
 // finalize_regridding! (dst ./= R.dst_areas), src/regridder/regrid.jl:95-118;
 // N-D fields loop 1-D slices (NDSliceLoop, regrid.jl:303-318).  Here a single
-// launch handles every field.  A is sparse (about 5 non-zeros per row), so the
-// apply is bound by HBM bandwidth, not by arithmetic: no MFMA.
+// launch handles every field.  A is sparse (about 5 non-zeros per row, ~0.25
+// flop per byte), so the apply is bound by HBM bandwidth: no MFMA.
 //
 // Kernels
-//   k_spmv_f64<G>         FP64 fields, G lanes per row, row gather (CSR-vector)
-//   k_spmm_f32_cells<G,R> FP32 fields with cells fastest ([n, k] column-major,
-//                         what regrid! receives with dims=1): G lanes per row keep
-//                         the row's (col, val) in registers and sweep the fields
-//   k_spmm_f32_fields     FP32 fields with fields fastest: one wave per row,
-//                         lanes across fields, fully coalesced loads
-// FP32 kernels form each product in FP64 (the FP64 A times the widened field
-// value), round it to FP32 and accumulate with compensated (Kahan-Babuska)
-// summation; the row is scaled by the FP64 reciprocal area.
+//   k_apply_rows<T,FB>  one thread per row; blockIdx.y = block of FB fields,
+//                       the slowest launch dimension; XCD-contiguous row tiles
+//   k_apply_long<T,FB>  rows with more than kLongRow non-zeros, one wave each
+//   k_apply_fieldsfast  FP32 fields with field stride 1: one wave per row,
+//                       lanes across fields, fully coalesced
+// FP64 fields accumulate in FP64 (fused multiply-add) and divide by the area
+// like finalize_regridding!.  FP32 fields use the FP64 A rounded to FP32 once
+// per non-zero and field block, FP32 products, FP32 Kahan-compensated row
+// sums, and a final scale by the FP64 reciprocal area.
 #include <algorithm>
+#include <vector>
 
 #include "regridder.hpp"
 
 namespace cr {
 
-// ---- FP32 compensated sums ------------------------------------------------
-struct CompF {
-    float s, c;
+constexpr int kApplyBlock = 256;
+constexpr int kLongRow = 32;
+
+// ---- accumulators ------------------------------------------------------------
+template <class T>
+struct Acc;
+
+template <>
+struct Acc<double> {
+    double s;
+    __device__ void init() { s = 0.0; }
+    __device__ void add(double v, double x) { s = fma(v, x, s); }
+    __device__ void merge_from_lane(int m) { s += __shfl_xor(s, m, 64); }
+    __device__ double finish(const double *area, const double *, int64_t row) const {
+        return area ? s / area[row] : s;
+    }
 };
-__device__ __forceinline__ void comp_add(CompF &a, float p) {
-    float t = a.s + p;
-    float bp = t - a.s;
-    float e = (a.s - (t - bp)) + (p - bp);
-    a.s = t;
-    a.c += e;
-}
-__device__ __forceinline__ CompF comp_merge(CompF a, CompF b) {
-    comp_add(a, b.s);
-    a.c += b.c;
-    return a;
-}
 
-template <int G>
-__device__ __forceinline__ double group_sum(double v) {
-#pragma unroll
-    for (int m = G / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, G);
-    return v;
-}
-
-template <int G>
-__device__ __forceinline__ CompF group_sum(CompF v) {
-#pragma unroll
-    for (int m = G / 2; m >= 1; m >>= 1) {
-        CompF o{__shfl_xor(v.s, m, G), __shfl_xor(v.c, m, G)};
-        v = comp_merge(v, o);
+template <>
+struct Acc<float> {
+    float s, c;  // Kahan: the exact sum is s - c
+    __device__ void init() {
+        s = 0.f;
+        c = 0.f;
     }
-    return v;
-}
-
-// ---- FP64 ------------------------------------------------------------------
-template <int G>
-__global__ void __launch_bounds__(256) k_spmv_f64(const int64_t *__restrict__ rowptr,
-                                                  const int32_t *__restrict__ col,
-                                                  const double *__restrict__ val, int64_t nrows,
-                                                  const double *__restrict__ x, int64_t xc, int64_t xf,
-                                                  double *__restrict__ y, int64_t yc, int64_t yf,
-                                                  int64_t nfields, const double *__restrict__ area) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t row = t / G;
-    int lane = (int)(t % G);
-    if (row >= nrows) return;  // whole groups exit together (G divides 64)
-    int64_t k0 = rowptr[row], k1 = rowptr[row + 1];
-    for (int64_t f = 0; f < nfields; ++f) {
-        const double *xs = x + f * xf;
-        double s = 0.0;
-        for (int64_t k = k0 + lane; k < k1; k += G) s += val[k] * xs[(int64_t)col[k] * xc];
-        s = group_sum<G>(s);
-        if (lane == 0) y[row * yc + f * yf] = area ? s / area[row] : s;
+    __device__ void add(float v, float x) {
+        float yk = v * x - c;
+        float t = s + yk;
+        c = (t - s) - yk;
+        s = t;
     }
-}
+    __device__ void merge_from_lane(int m) {
+        float os = __shfl_xor(s, m, 64), oc = __shfl_xor(c, m, 64);
+        // (s - c) + (os - oc) with the rounding error of s + os kept in c
+        float t = s + os;
+        float bp = t - s;
+        float e = (s - (t - bp)) + (os - bp);
+        s = t;
+        c = (c + oc) - e;
+    }
+    __device__ float finish(const double *, const double *inv, int64_t row) const {
+        double v = (double)s - (double)c;
+        return (float)(inv ? v * inv[row] : v);
+    }
+};
 
-// ---- FP32, cells fastest -----------------------------------------------------
-// G lanes per row (G = 1: one thread per row, no cross-lane reduction).  Rows
-// with at most G*R non-zeros keep their (offset, value) pairs in registers and
-// sweep the fields FB at a time, so FB*R independent gathers are in flight per
-// lane; longer rows stream (col, val) from cache for every field batch.
-template <int G, int R, int FB>
-__global__ void __launch_bounds__(256) k_spmm_f32_cells(
+template <class T>
+__device__ __forceinline__ T a_value(double v);
+template <>
+__device__ __forceinline__ double a_value<double>(double v) { return v; }
+template <>
+__device__ __forceinline__ float a_value<float>(double v) { return (float)v; }
+
+// ---- row kernel ------------------------------------------------------------------
+// One thread per row; blockIdx.y selects FB fields and is the slowest launch
+// dimension, so the whole GPU sweeps the same few fields at a time (their
+// slices of x stay in L2/MALL, few pages are live) while A is re-read from
+// cache once per field block.  Workgroups go round-robin to the 8 XCDs, so
+// tile = (bx % 8) * tiles_per_xcd + bx / 8 gives each XCD's L2 a contiguous
+// range of rows, whose source cells overlap.  Rows longer than kLongRow are
+// skipped here and done by k_apply_long.
+template <class T, int FB>
+__global__ void __launch_bounds__(kApplyBlock) k_apply_rows(
     const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
-    const double *__restrict__ val, int64_t nrows, const float *__restrict__ x, int64_t xc,
-    int64_t xf, float *__restrict__ y, int64_t yc, int64_t yf, int64_t nfields,
-    const double *__restrict__ inv_area) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t row = t / G;
-    int lane = (int)(t % G);
+    const double *__restrict__ val, int64_t nrows, const T *__restrict__ x, int64_t xc, int64_t xf,
+    T *__restrict__ y, int64_t yc, int64_t yf, int64_t nfields, const double *__restrict__ area,
+    const double *__restrict__ inv_area, int64_t tiles_per_xcd) {
+    const int64_t bx = blockIdx.x;
+    const int64_t tile = (bx & 7) * tiles_per_xcd + (bx >> 3);
+    const int64_t row = tile * kApplyBlock + threadIdx.x;
     if (row >= nrows) return;
-    int64_t k0 = rowptr[row], k1 = rowptr[row + 1];
-    double scale = inv_area ? inv_area[row] : 1.0;
-    float *yr = y + row * yc;
-    if (k1 - k0 <= (int64_t)G * R) {
-        int64_t off[R];
-        double v[R];
-        const int nmine = (int)((k1 - k0 - lane + G - 1) / G);  // entries this lane holds
+    const int64_t k0 = rowptr[row], k1 = rowptr[row + 1];
+    if (k1 - k0 > kLongRow) return;
+    const int64_t f0 = (int64_t)blockIdx.y * FB;
+    const int nb = (int)min((int64_t)FB, nfields - f0);
+    Acc<T> acc[FB];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            int64_t k = k0 + lane + (int64_t)r * G;
-            bool ok = r < nmine;
-            off[r] = ok ? (int64_t)col[k] * xc : 0;
-            v[r] = ok ? val[k] : 0.0;
-        }
-        for (int64_t f = 0; f < nfields; f += FB) {
-            CompF acc[FB];
+    for (int b = 0; b < FB; ++b) acc[b].init();
+    const T *xb = x + f0 * xf;
+    for (int64_t k = k0; k < k1; ++k) {
+        const T *xp = xb + (int64_t)col[k] * xc;
+        const T v = a_value<T>(val[k]);
 #pragma unroll
-            for (int b = 0; b < FB; ++b) acc[b] = CompF{0.f, 0.f};
+        for (int b = 0; b < FB; ++b)
+            if (FB == 1 || b < nb) acc[b].add(v, xp[b * xf]);
+    }
+    T *yp = y + row * yc + f0 * yf;
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                if (r < nmine) {
+    for (int b = 0; b < FB; ++b)
+        if (FB == 1 || b < nb) yp[b * yf] = acc[b].finish(area, inv_area, row);
+}
+
+// Rows longer than kLongRow: one wave per (row, field block); lanes split the
+// row; fixed-order butterfly reduction (deterministic).
+template <class T, int FB>
+__global__ void __launch_bounds__(64) k_apply_long(
+    const int32_t *__restrict__ long_rows, const int64_t *__restrict__ rowptr,
+    const int32_t *__restrict__ col, const double *__restrict__ val, const T *__restrict__ x,
+    int64_t xc, int64_t xf, T *__restrict__ y, int64_t yc, int64_t yf, int64_t nfields,
+    const double *__restrict__ area, const double *__restrict__ inv_area) {
+    const int64_t row = long_rows[blockIdx.x];
+    const int lane = threadIdx.x;
+    const int64_t k0 = rowptr[row], k1 = rowptr[row + 1];
+    const int64_t f0 = (int64_t)blockIdx.y * FB;
+    const int nb = (int)min((int64_t)FB, nfields - f0);
+    Acc<T> acc[FB];
 #pragma unroll
-                    for (int b = 0; b < FB; ++b)
-                        if (FB == 1 || f + b < nfields)
-                            comp_add(acc[b], (float)(v[r] * (double)x[off[r] + (f + b) * xf]));
-                }
-            }
+    for (int b = 0; b < FB; ++b) acc[b].init();
+    const T *xb = x + f0 * xf;
+    for (int64_t k = k0 + lane; k < k1; k += 64) {
+        const T *xp = xb + (int64_t)col[k] * xc;
+        const T v = a_value<T>(val[k]);
 #pragma unroll
-            for (int b = 0; b < FB; ++b) {
-                CompF a = group_sum<G>(acc[b]);
-                if (lane == 0 && (FB == 1 || f + b < nfields))
-                    yr[(f + b) * yf] = (float)(((double)a.s + (double)a.c) * scale);
-            }
-        }
-    } else {
-        for (int64_t f = 0; f < nfields; ++f) {
-            const float *xs = x + f * xf;
-            CompF acc{0.f, 0.f};
-            for (int64_t k = k0 + lane; k < k1; k += G)
-                comp_add(acc, (float)(val[k] * (double)xs[(int64_t)col[k] * xc]));
-            acc = group_sum<G>(acc);
-            if (lane == 0) yr[f * yf] = (float)(((double)acc.s + (double)acc.c) * scale);
-        }
+        for (int b = 0; b < FB; ++b)
+            if (FB == 1 || b < nb) acc[b].add(v, xp[b * xf]);
+    }
+#pragma unroll
+    for (int b = 0; b < FB; ++b)
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) acc[b].merge_from_lane(m);
+    if (lane == 0) {
+        T *yp = y + row * yc + f0 * yf;
+#pragma unroll
+        for (int b = 0; b < FB; ++b)
+            if (FB == 1 || b < nb) yp[b * yf] = acc[b].finish(area, inv_area, row);
     }
 }
 
-// ---- FP32, fields fastest (field stride 1) -----------------------------------
-// One wave per row; lane l owns fields l, l+64, l+128, ... (up to kFieldRegs).
-constexpr int kFieldRegs = 8;
+// ---- FP32, fields fastest (field stride 1) ----------------------------------------
+// One wave per row; lane l owns fields l, l+64, ... (up to kFieldRegs per lane).
+constexpr int kFieldRegs = 4;
 
-__global__ void __launch_bounds__(256) k_spmm_f32_fields(
+__global__ void __launch_bounds__(256) k_apply_fieldsfast(
     const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
     const double *__restrict__ val, int64_t nrows, const float *__restrict__ x, int64_t xc,
     float *__restrict__ y, int64_t yc, int64_t nfields, const double *__restrict__ inv_area) {
@@ -151,24 +163,23 @@ __global__ void __launch_bounds__(256) k_spmm_f32_fields(
     int lane = threadIdx.x & 63;
     if (row >= nrows) return;
     int64_t k0 = rowptr[row], k1 = rowptr[row + 1];
-    double scale = inv_area ? inv_area[row] : 1.0;
     for (int64_t f0 = 0; f0 < nfields; f0 += 64 * kFieldRegs) {
-        CompF acc[kFieldRegs];
+        Acc<float> acc[kFieldRegs];
 #pragma unroll
-        for (int j = 0; j < kFieldRegs; ++j) acc[j] = CompF{0.f, 0.f};
+        for (int j = 0; j < kFieldRegs; ++j) acc[j].init();
         for (int64_t k = k0; k < k1; ++k) {
             const float *xr = x + (int64_t)col[k] * xc;
-            double v = val[k];
+            float v = (float)val[k];
 #pragma unroll
             for (int j = 0; j < kFieldRegs; ++j) {
                 int64_t f = f0 + lane + 64 * j;
-                if (f < nfields) comp_add(acc[j], (float)(v * (double)xr[f]));
+                if (f < nfields) acc[j].add(v, xr[f]);
             }
         }
 #pragma unroll
         for (int j = 0; j < kFieldRegs; ++j) {
             int64_t f = f0 + lane + 64 * j;
-            if (f < nfields) y[row * yc + f] = (float)(((double)acc[j].s + (double)acc[j].c) * scale);
+            if (f < nfields) y[row * yc + f] = acc[j].finish(nullptr, inv_area, row);
         }
     }
 }
@@ -176,6 +187,39 @@ __global__ void __launch_bounds__(256) k_spmm_f32_fields(
 __global__ void k_inv(const double *__restrict__ a, int64_t n, double *__restrict__ inv) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) inv[i] = 1.0 / a[i];
+}
+
+// Rows with more than kLongRow non-zeros, appended in any order (each long row
+// is reduced by its own wave, so the list order does not affect results).
+__global__ void k_long_rows(const int64_t *__restrict__ rowptr, int64_t n, int32_t *__restrict__ list,
+                            unsigned long long *__restrict__ stat) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long nnz = (unsigned long long)(rowptr[i + 1] - rowptr[i]);
+    if (nnz > (unsigned long long)kLongRow) list[atomicAdd(&stat[0], 1ull)] = (int32_t)i;
+    atomicMax(&stat[1], nnz);
+}
+
+static void plan_long_rows(Csr &A, hipStream_t s) {
+    A.long_rows.release();
+    A.nlong = 0;
+    A.max_row_nnz = 0;
+    if (A.nrows == 0) return;
+    DevBuf<int32_t> list(A.nrows, s);
+    DevBuf<unsigned long long> stat(2, s);
+    CR_HIP(hipMemsetAsync(stat.p, 0, stat.bytes(), s));
+    hipLaunchKernelGGL(k_long_rows, dim3(grid_for(A.nrows, 256)), dim3(256), 0, s, A.rowptr.p, A.nrows,
+                       list.p, stat.p);
+    CR_HIP(hipGetLastError());
+    unsigned long long h[2] = {0, 0};
+    CR_HIP(hipMemcpyAsync(h, stat.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    CR_HIP(hipStreamSynchronize(s));
+    A.nlong = (int64_t)h[0];
+    A.max_row_nnz = (int64_t)h[1];
+    if (A.nlong) {
+        A.long_rows.alloc(A.nlong);
+        CR_HIP(hipMemcpyAsync(A.long_rows.p, list.p, A.nlong * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    }
 }
 
 void finalize_regridder(cr_regridder *R, hipStream_t s) {
@@ -186,60 +230,68 @@ void finalize_regridder(cr_regridder *R, hipStream_t s) {
     hipLaunchKernelGGL(k_inv, dim3(grid_for(R->bwd.nrows, 256)), dim3(256), 0, s, R->a_src.p,
                        R->bwd.nrows, R->bwd.inv_area.p);
     CR_HIP(hipGetLastError());
+    plan_long_rows(R->fwd, s);
+    plan_long_rows(R->bwd, s);
 }
 
-// Apply variants of the FP32 cells-fastest kernel (the autotuner's search space).
-struct CellsVariant {
-    int G, R, FB;
-};
-constexpr CellsVariant kCellsVariants[] = {{1, 8, 1}, {1, 8, 2}, {1, 8, 4}, {2, 4, 2}, {4, 4, 1}, {4, 4, 4}};
-constexpr int kNumCellsVariants = sizeof(kCellsVariants) / sizeof(kCellsVariants[0]);
-constexpr int kDefaultCellsVariant = 1;
-
-template <int G, int R, int FB>
-void launch_cells(const Csr &A, const float *x, int64_t sc, int64_t sf, float *y, int64_t dc,
-                  int64_t df, int64_t nf, const double *inv, hipStream_t s) {
-    const int B = 256;
-    hipLaunchKernelGGL((k_spmm_f32_cells<G, R, FB>), dim3(grid_for(A.nrows * G, B)), dim3(B), 0, s,
-                       A.rowptr.p, A.col.p, A.val.p, A.nrows, x, sc, sf, y, dc, df, nf, inv);
+// ---- launch ------------------------------------------------------------------------
+template <class T, int FB>
+static void launch_rows(const Csr &A, const T *x, int64_t sc, int64_t sf, T *y, int64_t dc,
+                        int64_t df, int64_t nf, const double *area, const double *inv, hipStream_t s) {
+    int64_t ntiles = (A.nrows + kApplyBlock - 1) / kApplyBlock;
+    int64_t tiles_per_xcd = (ntiles + 7) / 8;
+    int64_t gy = (nf + FB - 1) / FB;
+    CR_REQUIRE(gy <= 65535, CR_UNSUPPORTED, "too many fields for one launch");
+    hipLaunchKernelGGL((k_apply_rows<T, FB>), dim3((unsigned)(tiles_per_xcd * 8), (unsigned)gy),
+                       dim3(kApplyBlock), 0, s, A.rowptr.p, A.col.p, A.val.p, A.nrows, x, sc, sf, y, dc,
+                       df, nf, area, inv, tiles_per_xcd);
+    if (A.nlong)
+        hipLaunchKernelGGL((k_apply_long<T, FB>), dim3((unsigned)A.nlong, (unsigned)gy), dim3(64), 0, s,
+                           A.long_rows.p, A.rowptr.p, A.col.p, A.val.p, x, sc, sf, y, dc, df, nf, area,
+                           inv);
 }
 
-void launch_cells_variant(int v, const Csr &A, const float *x, int64_t sc, int64_t sf, float *y,
-                          int64_t dc, int64_t df, int64_t nf, const double *inv, hipStream_t s) {
+// Field-block widths the autotuner tries for FP32 fields (index = variant).
+constexpr int kFieldBlocks[] = {1, 2, 4, 8, 16};
+constexpr int kNumVariants = sizeof(kFieldBlocks) / sizeof(kFieldBlocks[0]);
+constexpr int kDefaultF32Variant = 3;  // FB = 8
+
+static void launch_f32_variant(int v, const Csr &A, const float *x, int64_t sc, int64_t sf, float *y,
+                               int64_t dc, int64_t df, int64_t nf, const double *inv, hipStream_t s) {
     switch (v) {
-        case 0: launch_cells<1, 8, 1>(A, x, sc, sf, y, dc, df, nf, inv, s); break;
-        case 1: launch_cells<1, 8, 2>(A, x, sc, sf, y, dc, df, nf, inv, s); break;
-        case 2: launch_cells<1, 8, 4>(A, x, sc, sf, y, dc, df, nf, inv, s); break;
-        case 3: launch_cells<2, 4, 2>(A, x, sc, sf, y, dc, df, nf, inv, s); break;
-        case 4: launch_cells<4, 4, 1>(A, x, sc, sf, y, dc, df, nf, inv, s); break;
-        default: launch_cells<4, 4, 4>(A, x, sc, sf, y, dc, df, nf, inv, s); break;
+        case 0: launch_rows<float, 1>(A, x, sc, sf, y, dc, df, nf, nullptr, inv, s); break;
+        case 1: launch_rows<float, 2>(A, x, sc, sf, y, dc, df, nf, nullptr, inv, s); break;
+        case 2: launch_rows<float, 4>(A, x, sc, sf, y, dc, df, nf, nullptr, inv, s); break;
+        case 3: launch_rows<float, 8>(A, x, sc, sf, y, dc, df, nf, nullptr, inv, s); break;
+        default: launch_rows<float, 16>(A, x, sc, sf, y, dc, df, nf, nullptr, inv, s); break;
     }
 }
 
-// Kernel choice for a call: FP64 -> CSR-vector; FP32 with unit field strides
-// and several fields -> fields-fastest kernel; otherwise the cells-fastest
-// variant the autotuner picked for this direction (or the default).
+// Kernel choice: FP64 -> row kernel (1 field per block for a single field, 8
+// otherwise); FP32 with unit field strides and many fields -> fields-fastest
+// kernel; other FP32 layouts -> the row kernel with the autotuned (or default)
+// field-block width.
 void apply(const cr_regridder *R, bool transpose, int dtype, int64_t nf, const void *src,
            int64_t sc, int64_t sf, void *dst, int64_t dc, int64_t df, bool normalize, hipStream_t s,
            int force_variant = -1) {
     const Csr &A = transpose ? R->bwd : R->fwd;
     const double *area = transpose ? R->a_src.p : R->a_dst.p;
     if (A.nrows == 0 || nf == 0) return;
-    const int B = 256;
     if (dtype == CR_F64) {
-        constexpr int G = 4;
-        hipLaunchKernelGGL(k_spmv_f64<G>, dim3(grid_for(A.nrows * G, B)), dim3(B), 0, s, A.rowptr.p,
-                           A.col.p, A.val.p, A.nrows, (const double *)src, sc, sf, (double *)dst, dc, df,
-                           nf, normalize ? area : nullptr);
-    } else if (sf == 1 && df == 1 && nf > 1) {
-        hipLaunchKernelGGL(k_spmm_f32_fields, dim3(grid_for(A.nrows, B / 64)), dim3(B), 0, s, A.rowptr.p,
+        const double *x = (const double *)src;
+        double *y = (double *)dst;
+        const double *ar = normalize ? area : nullptr;
+        if (nf == 1) launch_rows<double, 1>(A, x, sc, sf, y, dc, df, nf, ar, nullptr, s);
+        else launch_rows<double, 8>(A, x, sc, sf, y, dc, df, nf, ar, nullptr, s);
+    } else if (sf == 1 && df == 1 && nf >= 64 && force_variant < 0) {
+        hipLaunchKernelGGL(k_apply_fieldsfast, dim3(grid_for(A.nrows, 4)), dim3(256), 0, s, A.rowptr.p,
                            A.col.p, A.val.p, A.nrows, (const float *)src, sc, (float *)dst, dc, nf,
                            normalize ? A.inv_area.p : nullptr);
     } else {
-        int v = force_variant >= 0 ? force_variant : A.tuned_cells_variant;
-        if (v < 0 || v >= kNumCellsVariants) v = kDefaultCellsVariant;
-        launch_cells_variant(v, A, (const float *)src, sc, sf, (float *)dst, dc, df, nf,
-                             normalize ? A.inv_area.p : nullptr, s);
+        int v = force_variant >= 0 ? force_variant : A.tuned_variant;
+        if (v < 0 || v >= kNumVariants) v = kDefaultF32Variant;
+        launch_f32_variant(v, A, (const float *)src, sc, sf, (float *)dst, dc, df, nf,
+                           normalize ? A.inv_area.p : nullptr, s);
     }
     CR_HIP(hipGetLastError());
 }
@@ -285,7 +337,7 @@ cr_status cr_regrid_tune(cr_regridder *R, int32_t transpose, int64_t nfields, co
         CR_HIP(hipEventCreate(&e1));
         int best = -1;
         float best_t = 1e30f;
-        for (int v = 0; v < kNumCellsVariants; ++v) {
+        for (int v = 0; v < kNumVariants; ++v) {
             // one warm-up launch, then the median of three timed launches
             apply(R, transpose != 0, CR_F32, nfields, src, src_cell_stride, src_field_stride, dst,
                   dst_cell_stride, dst_field_stride, true, s, v);
@@ -299,11 +351,14 @@ cr_status cr_regrid_tune(cr_regridder *R, int32_t transpose, int64_t nfields, co
                 CR_HIP(hipEventElapsedTime(&t[rep], e0, e1));
             }
             std::sort(t, t + 3);
-            if (t[1] < best_t) { best_t = t[1]; best = v; }
+            if (t[1] < best_t) {
+                best_t = t[1];
+                best = v;
+            }
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        A.tuned_cells_variant = best;
+        A.tuned_variant = best;
         if (chosen) *chosen = best;
         if (best_ms) *best_ms = best_t;
     });

@@ -266,9 +266,11 @@ struct Timer {
 
 // hipcub temporary storage, grown on demand.
 struct Temp {
+    hipStream_t s;
     DevBuf<unsigned char> buf;
+    explicit Temp(hipStream_t st) : s(st) {}
     void *get(size_t bytes) {
-        if (buf.n < bytes) buf.alloc(std::max(bytes, (size_t)1));
+        if (buf.n < bytes) buf.alloc_async(std::max(bytes, (size_t)256), s);
         return buf.p;
     }
 };
@@ -302,7 +304,7 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     const int64_t nd = dst->ncells, ns = src->ncells;
     const int B = 256;
     Timer tm(s);
-    Temp T;
+    Temp T(s);
     tm.mark();  // 0
 
     // clip side: a convex side clips; prefer the destination (GO clips p1=src by p2=dst)
@@ -318,7 +320,7 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
 
     // 1. areas + caps
     DevBuf<double> a_dst(nd), a_src(ns);
-    DevBuf<float4> cap_dst(nd), cap_src(ns);
+    DevBuf<float4> cap_dst(nd, s), cap_src(ns, s);
     hipLaunchKernelGGL(k_cell_bounds, dim3(grid_for(nd, B)), dim3(B), 0, s, dst->cells.p, nd, dst->nv,
                        a_dst.p, cap_dst.p);
     hipLaunchKernelGGL(k_cell_bounds, dim3(grid_for(ns, B)), dim3(B), 0, s, src->cells.p, ns, src->nv,
@@ -329,11 +331,11 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     // 2. tree over source caps
     Levels lv = make_levels(ns);
     int64_t nnodes = lv.off[lv.top] + 1;
-    DevBuf<float4> nodes(nnodes);
-    DevBuf<int32_t> perm(ns);
+    DevBuf<float4> nodes(nnodes, s);
+    DevBuf<int32_t> perm(ns, s);
     {
-        DevBuf<uint32_t> key(ns), key_sorted(ns);
-        DevBuf<int32_t> idx(ns);
+        DevBuf<uint32_t> key(ns, s), key_sorted(ns, s);
+        DevBuf<int32_t> idx(ns, s);
         hipLaunchKernelGGL(k_morton, dim3(grid_for(ns, B)), dim3(B), 0, s, cap_src.p, ns, key.p, idx.p);
         sort_pairs(T, key.p, key_sorted.p, idx.p, perm.p, ns, 30, s);
         hipLaunchKernelGGL(k_gather_caps, dim3(grid_for(ns, B)), dim3(B), 0, s, cap_src.p, perm.p, ns,
@@ -346,9 +348,9 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     tm.mark();  // 2
 
     // 3. candidate search: count, scan, fill
-    DevBuf<int64_t> cand_off(nd + 1);
+    DevBuf<int64_t> cand_off(nd + 1, s);
     {
-        DevBuf<int64_t> cand_cnt(nd + 1);
+        DevBuf<int64_t> cand_cnt(nd + 1, s);
         CR_HIP(hipMemsetAsync(cand_cnt.p + nd, 0, sizeof(int64_t), s));
         hipLaunchKernelGGL(k_traverse<false>, dim3(grid_for(nd, B)), dim3(B), 0, s, cap_dst.p, nd,
                            nodes.p, lv, perm.p, cand_cnt.p, (int32_t *)nullptr, (int32_t *)nullptr);
@@ -359,7 +361,7 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     CR_HIP(hipMemcpyAsync(&ncand, cand_off.p + nd, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     CR_HIP(hipStreamSynchronize(s));
     CR_REQUIRE(ncand < (int64_t(1) << 31), CR_UNSUPPORTED, "more than 2^31 candidate pairs");
-    DevBuf<int32_t> cand_src(ncand), cand_dst(ncand);
+    DevBuf<int32_t> cand_src(ncand, s), cand_dst(ncand, s);
     if (ncand) {  // zero-fill: any slot the fill pass skipped is the valid pair (0, 0)
         CR_HIP(hipMemsetAsync(cand_src.p, 0, cand_src.bytes(), s));
         CR_HIP(hipMemsetAsync(cand_dst.p, 0, cand_dst.bytes(), s));
@@ -371,7 +373,7 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     tm.mark();  // 3
 
     // 4. clip + area per candidate
-    DevBuf<double> cval(ncand);
+    DevBuf<double> cval(ncand, s);
     if (ncand) {
         if (clip_nv <= 4)
             hipLaunchKernelGGL(k_clip<4>, dim3(grid_for(ncand, B)), dim3(B), 0, s, cand_dst.p,
@@ -387,9 +389,9 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
 
     // 5. compaction (row-grouped COO) and the two CSRs
     int64_t nblk = (ncand + kCompactBlock - 1) / kCompactBlock;
-    DevBuf<int64_t> blk(nblk + 1);
+    DevBuf<int64_t> blk(nblk + 1, s);
     {
-        DevBuf<int64_t> blk_cnt(nblk + 1);
+        DevBuf<int64_t> blk_cnt(nblk + 1, s);
         CR_HIP(hipMemsetAsync(blk_cnt.p, 0, blk_cnt.bytes(), s));
         if (ncand)
             hipLaunchKernelGGL(k_keep_count, dim3((unsigned)nblk), dim3(kCompactBlock), 0, s, cval.p,
@@ -409,9 +411,9 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     F.col.alloc(nnz); F.val.alloc(nnz);
     Bk.col.alloc(nnz); Bk.val.alloc(nnz);
     {
-        DevBuf<int32_t> coo_row(nnz), coo_col(nnz);
-        DevBuf<double> coo_val(nnz);
-        DevBuf<unsigned long long> rcnt(nd + 1), ccnt(ns + 1);
+        DevBuf<int32_t> coo_row(nnz, s), coo_col(nnz, s);
+        DevBuf<double> coo_val(nnz, s);
+        DevBuf<unsigned long long> rcnt(nd + 1, s), ccnt(ns + 1, s);
         CR_HIP(hipMemsetAsync(rcnt.p, 0, rcnt.bytes(), s));
         CR_HIP(hipMemsetAsync(ccnt.p, 0, ccnt.bytes(), s));
         if (ncand)
@@ -424,7 +426,7 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
         exclusive_sum(T, (int64_t *)ccnt.p, Bk.rowptr.p, ns + 1, s);
         if (nnz) {
             // stable sort by column: CSR(A^T), rows of A^T sorted by A's row
-            DevBuf<int32_t> iota(nnz), p1(nnz), tkey(nnz), p2(nnz), fkey(nnz);
+            DevBuf<int32_t> iota(nnz, s), p1(nnz, s), tkey(nnz, s), p2(nnz, s), fkey(nnz, s);
             hipLaunchKernelGGL(k_iota, dim3(grid_for(nnz, B)), dim3(B), 0, s, iota.p, nnz);
             sort_pairs(T, coo_col.p, tkey.p, iota.p, p1.p, nnz, bits_for(ns), s);
             hipLaunchKernelGGL(k_gather<int32_t>, dim3(grid_for(nnz, B)), dim3(B), 0, s, coo_row.p, p1.p,
@@ -475,8 +477,8 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
 void normalize_regridder(cr_regridder *R, hipStream_t s) {
     Csr &F = R->fwd;
     if (F.nnz == 0) return;
-    Temp T;
-    DevBuf<double> mx(1);
+    Temp T(s);
+    DevBuf<double> mx(1, s);
     size_t bytes = 0;
     CR_HIP(hipcub::DeviceReduce::Max(nullptr, bytes, F.val.p, mx.p, (int)F.nnz, s));
     void *p = T.get(bytes);

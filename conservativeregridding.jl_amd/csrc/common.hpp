@@ -52,28 +52,55 @@ cr_status guard(F &&f) {
     }
 }
 
-// Device memory owned by a handle.
+// Device memory.  DevBuf(n) uses hipMalloc (long-lived buffers owned by a
+// handle); DevBuf(n, stream) takes stream-ordered memory from the device's
+// default pool (build temporaries), which keeps freed blocks for the next
+// build instead of returning them to the driver.
+void configure_pool();
+
 template <class T>
 struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
+    hipStream_t st = nullptr;
+    bool pooled = false;
     DevBuf() = default;
     explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(size_t count, hipStream_t s) { alloc_async(count, s); }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n), st(o.st), pooled(o.pooled) {
+        o.p = nullptr;
+        o.n = 0;
+    }
     DevBuf &operator=(DevBuf &&o) noexcept {
-        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        if (this != &o) {
+            release();
+            p = o.p; n = o.n; st = o.st; pooled = o.pooled;
+            o.p = nullptr; o.n = 0;
+        }
         return *this;
     }
     ~DevBuf() { release(); }
     void alloc(size_t count) {
         release();
         n = count;
+        pooled = false;
         if (count) CR_HIP(hipMalloc(&p, count * sizeof(T)));
     }
+    void alloc_async(size_t count, hipStream_t s) {
+        release();
+        n = count;
+        st = s;
+        pooled = true;
+        configure_pool();
+        if (count) CR_HIP(hipMallocAsync((void **)&p, count * sizeof(T), s));
+    }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (pooled) (void)hipFreeAsync(p, st);
+            else (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
     }

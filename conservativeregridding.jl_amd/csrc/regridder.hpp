@@ -14,23 +14,16 @@
 
 namespace cr {
 
-// Row-block plan of one CSR for the batched apply: block b covers rows
-// [row0[b], row0[b+1]).  Blocks hold at most kPlanRows rows and, unless a
-// single row is longer, at most kPlanNnz non-zeros.
-struct ApplyPlan {
-    int64_t nblocks = 0;
-    DevBuf<int32_t> row0;  // nblocks + 1
-    int32_t max_block_nnz = 0;
-};
-
 struct Csr {
     int64_t nrows = 0, ncols = 0, nnz = 0;
     DevBuf<int64_t> rowptr;
     DevBuf<int32_t> col;
     DevBuf<double> val;
-    DevBuf<double> inv_area;  // 1 / area of the row cells (FP32 batched path)
-    ApplyPlan plan;
-    int tuned_cells_variant = -1;  // set by cr_regrid_tune; -1: default
+    DevBuf<double> inv_area;   // 1 / area of the row cells (FP32 batched path)
+    DevBuf<int32_t> long_rows; // rows with more than kLongRow non-zeros (own kernel)
+    int64_t nlong = 0;
+    int64_t max_row_nnz = 0;
+    int tuned_variant = -1;    // FP32 field-block width chosen by cr_regrid_tune; -1: default
 };
 
 }  // namespace cr
