@@ -62,6 +62,7 @@ SIGNATURES = {
     "cr_grid_from_points": (c_i32, [ctypes.POINTER(c_vp), _dp, c_i64, c_i64, c_i64, c_vp]),
     "cr_grid_healpix": (c_i32, [ctypes.POINTER(c_vp), c_i64, c_i32, c_i32, c_vp]),
     "cr_grid_info": (c_i32, [c_vp, _i64p, _i32p, _i32p]),
+    "cr_grid_vertex_count": (c_i32, [c_vp, _i64p]),
     "cr_grid_copy_cells": (c_i32, [c_vp, _dp]),
     "cr_grid_areas": (c_i32, [c_vp, _dp]),
     "cr_grid_destroy": (None, [c_vp]),
@@ -77,6 +78,8 @@ SIGNATURES = {
     "cr_regridder_destroy": (None, [c_vp]),
     "cr_regrid": (c_i32, [c_vp, c_i32, c_i32, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, c_i32,
                           c_vp]),
+    "cr_regrid_backward": (c_i32, [c_vp, c_vp, c_vp, c_i32, c_i32, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64, c_i64,
+                                   c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp]),
     "cr_regrid_tune": (c_i32, [c_vp, c_i32, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, _i32p,
                                ctypes.POINTER(ctypes.c_float), c_vp]),
 }

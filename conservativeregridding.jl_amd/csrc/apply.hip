@@ -217,14 +217,14 @@ static void plan_long_rows(Csr &A, hipStream_t s) {
     A.nlong = (int64_t)h[0];
     A.max_row_nnz = (int64_t)h[1];
     if (A.nlong) {
-        A.long_rows.alloc(A.nlong);
+        A.long_rows.alloc_kept(A.nlong, s);
         CR_HIP(hipMemcpyAsync(A.long_rows.p, list.p, A.nlong * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
     }
 }
 
 void finalize_regridder(cr_regridder *R, hipStream_t s) {
-    R->fwd.inv_area.alloc(R->fwd.nrows);
-    R->bwd.inv_area.alloc(R->bwd.nrows);
+    R->fwd.inv_area.alloc_kept(R->fwd.nrows, s);
+    R->bwd.inv_area.alloc_kept(R->bwd.nrows, s);
     hipLaunchKernelGGL(k_inv, dim3(grid_for(R->fwd.nrows, 256)), dim3(256), 0, s, R->a_dst.p,
                        R->fwd.nrows, R->fwd.inv_area.p);
     hipLaunchKernelGGL(k_inv, dim3(grid_for(R->bwd.nrows, 256)), dim3(256), 0, s, R->a_src.p,

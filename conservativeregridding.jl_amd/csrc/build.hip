@@ -319,7 +319,9 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     const int clip_nv = clip_is_dst ? dst->nv : src->nv;
 
     // 1. areas + caps
-    DevBuf<double> a_dst(nd), a_src(ns);
+    DevBuf<double> a_dst, a_src;
+    a_dst.alloc_kept(nd, s);
+    a_src.alloc_kept(ns, s);
     DevBuf<float4> cap_dst(nd, s), cap_src(ns, s);
     hipLaunchKernelGGL(k_cell_bounds, dim3(grid_for(nd, B)), dim3(B), 0, s, dst->cells.p, nd, dst->nv,
                        a_dst.p, cap_dst.p);
@@ -406,10 +408,12 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     Csr &Bk = R->bwd;
     F.nrows = nd; F.ncols = ns; F.nnz = nnz;
     Bk.nrows = ns; Bk.ncols = nd; Bk.nnz = nnz;
-    F.rowptr.alloc(nd + 1);
-    Bk.rowptr.alloc(ns + 1);
-    F.col.alloc(nnz); F.val.alloc(nnz);
-    Bk.col.alloc(nnz); Bk.val.alloc(nnz);
+    F.rowptr.alloc_kept(nd + 1, s);
+    Bk.rowptr.alloc_kept(ns + 1, s);
+    F.col.alloc_kept(nnz, s);
+    F.val.alloc_kept(nnz, s);
+    Bk.col.alloc_kept(nnz, s);
+    Bk.val.alloc_kept(nnz, s);
     {
         DevBuf<int32_t> coo_row(nnz, s), coo_col(nnz, s);
         DevBuf<double> coo_val(nnz, s);
@@ -454,6 +458,7 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
         hipLaunchKernelGGL(k_scale, dim3(grid_for(ns, B)), dim3(B), 0, s, R->a_src.p, ns, r2, 1.0);
         CR_HIP(hipGetLastError());
     }
+    R->area_scale = r2;
     finalize_regridder(R, s);
     tm.mark();  // 5
     CR_HIP(hipStreamSynchronize(s));
@@ -494,6 +499,7 @@ void normalize_regridder(cr_regridder *R, hipStream_t s) {
     hipLaunchKernelGGL(k_scale, dim3(grid_for(F.nrows, B)), dim3(B), 0, s, R->a_dst.p, F.nrows, 1.0, m);
     hipLaunchKernelGGL(k_scale, dim3(grid_for(F.ncols, B)), dim3(B), 0, s, R->a_src.p, F.ncols, 1.0, m);
     CR_HIP(hipGetLastError());
+    R->area_scale /= m;
     finalize_regridder(R, s);
     CR_HIP(hipStreamSynchronize(s));
 }

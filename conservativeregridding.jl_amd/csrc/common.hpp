@@ -96,6 +96,12 @@ struct DevBuf {
         configure_pool();
         if (count) CR_HIP(hipMallocAsync((void **)&p, count * sizeof(T), s));
     }
+    // Pool memory that outlives the call that allocated it (owned by a handle):
+    // allocated in order on `s`, released on the null stream.
+    void alloc_kept(size_t count, hipStream_t s) {
+        alloc_async(count, s);
+        st = nullptr;
+    }
     void release() {
         if (p) {
             if (pooled) (void)hipFreeAsync(p, st);
@@ -128,5 +134,7 @@ struct cr_grid {
     int32_t nv = 0;
     int32_t convex = 0;
     int32_t flipped = 0;
-    cr::DevBuf<double> cells;  // ncells * nv * 3, xyz interleaved per vertex
+    int64_t nverts = 0;          // caller-visible vertices (points of a point grid)
+    cr::DevBuf<double> cells;    // ncells * nv * 3, xyz interleaved per vertex
+    cr::DevBuf<int32_t> vidx;    // ncells * nv: caller vertex of each cell slot
 };

@@ -1,0 +1,202 @@
+// grad.hip -- reverse mode of regrid!(y, R, x) w.r.t. the field and the grid
+// vertices (no reference counterpart: the reference has no AD; this is what a
+// ChainRulesCore rrule around regrid! calls).
+//
+// With y = (A x) ./ a_to (normalize) and upstream gradient gy:
+//   dL/dx          = A^T (gy ./ a_to)                      (the transpose apply)
+//   dL/dA_ij       = sum_f gy_if x_jf / a_to,i             (per intersection pair)
+//   dL/da_to,i     = -sum_f gy_if y_if / a_to,i            (per destination cell)
+// and every A_ij, a_to,i is an area of unit-sphere polygons, so the vertex
+// gradient is the sum over pairs of dL/dA_ij * d(area of cell_i ∩ cell_j)/dV
+// plus the sum over cells of dL/da_to,i * d(area of cell_i)/dV, scattered into
+// per-vertex gradients with FP64 atomics (order-nondeterministic in the last
+// bits).  Gradients are tangential: the areas depend on the vertices through
+// v/|v|.  Areas here are on the unit sphere; `area_scale` converts.
+#include "regridder.hpp"
+#include "sphgeom.hpp"
+
+namespace cr {
+
+// Per non-zero of CSR(A) (rows = dst, cols = src): the coefficient dL/dA_u,ij
+// (unit-sphere areas) and the row index.
+__global__ void k_pair_coef(const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                            int64_t nrows, int transpose, int normalize, double kappa,
+                            const double *__restrict__ a_to, int64_t nf,
+                            const double *__restrict__ x, int64_t xc, int64_t xf,
+                            const double *__restrict__ gy, int64_t gc, int64_t gf,
+                            double *__restrict__ coef, int32_t *__restrict__ rowid) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows) return;
+    for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+        int64_t j = col[k];
+        // forward: y lives on dst rows i, x on src cols j; transpose: the reverse
+        int64_t to = transpose ? j : i, from = transpose ? i : j;
+        double s = 0.0;
+        for (int64_t f = 0; f < nf; ++f) s += gy[to * gc + f * gf] * x[from * xc + f * xf];
+        coef[k] = normalize ? kappa * s / a_to[to] : kappa * s;
+        rowid[k] = (int32_t)i;
+    }
+}
+
+// Scatter target of the clip backward: caller-vertex gradients of both grids.
+struct VertexSink {
+    const double *subj, *clip;
+    int ns, nc;
+    const int32_t *subj_vi, *clip_vi;  // caller vertex of each slot of the two cells
+    double *g_subj, *g_clip;           // caller-vertex gradients (may be null)
+
+    __device__ d3 sv(int e) const { return load3(subj + 3 * (e % ns)); }
+    __device__ d3 cv(int k) const {
+        int kk = k % 4;
+        return load3(clip + 3 * (kk < nc ? kk : nc - 1));
+    }
+    __device__ static void add(double *g, int32_t v, const d3 &d) {
+        if (!g) return;
+        atomicAdd(g + 3 * (int64_t)v, d.x);
+        atomicAdd(g + 3 * (int64_t)v + 1, d.y);
+        atomicAdd(g + 3 * (int64_t)v + 2, d.z);
+    }
+    __device__ void add_subj(int e, const d3 &g) const {
+        e %= ns;
+        add(g_subj, subj_vi[e], tangential(g, sv(e)));
+    }
+    __device__ void add_clip(int k, const d3 &g) const {
+        int kk = k % 4;
+        kk = kk < nc ? kk : nc - 1;
+        add(g_clip, clip_vi[kk], tangential(g, cv(kk)));
+    }
+    __device__ void operator()(int32_t l, const d3 &x, const d3 &g) const {
+        if (l < 0) return;
+        int32_t t = lab_type(l), a = lab_hi(l), b = lab_lo(l);
+        if (t == kLabS) {
+            add_subj(l & kLabIndexMask, g);  // the subject index is the low 30 bits
+        } else if (t == kLabC) {
+            add_clip(b, g);
+        } else {
+            d3 p1, q1, p2 = cv(b), q2 = cv(b + 1), gp1, gq1, gp2, gq2;
+            if (t == kLabXF) { p1 = sv(a); q1 = sv(a + 1); }
+            else { p1 = cv(a); q1 = cv(a + 1); }
+            if (!crossing_backward(x, g, p1, q1, p2, q2, gp1, gq1, gp2, gq2)) return;
+            if (t == kLabXF) { add_subj(a, gp1); add_subj(a + 1, gq1); }
+            else { add_clip(a, gp1); add_clip(a + 1, gq1); }
+            add_clip(b, gp2);
+            add_clip(b + 1, gq2);
+        }
+    }
+};
+
+__global__ void __launch_bounds__(256) k_clip_grad(
+    const int32_t *__restrict__ rowid, const int32_t *__restrict__ col, const double *__restrict__ coef,
+    int64_t nnz, const double *__restrict__ dst_cells, int dst_nv, const int32_t *__restrict__ dst_vi,
+    const double *__restrict__ src_cells, int src_nv, const int32_t *__restrict__ src_vi,
+    int clip_is_dst, double *__restrict__ g_dst, double *__restrict__ g_src) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    double c = coef[k];
+    if (c == 0.0) return;
+    int64_t d = rowid[k], s = col[k];
+    const double *dp = dst_cells + 3 * (int64_t)dst_nv * d;
+    const double *sp = src_cells + 3 * (int64_t)src_nv * s;
+    VertexSink sink;
+    if (clip_is_dst) {
+        sink = VertexSink{sp, dp, src_nv, dst_nv, src_vi + (int64_t)src_nv * s, dst_vi + (int64_t)dst_nv * d,
+                          g_src, g_dst};
+    } else {
+        sink = VertexSink{dp, sp, dst_nv, src_nv, dst_vi + (int64_t)dst_nv * d, src_vi + (int64_t)src_nv * s,
+                          g_dst, g_src};
+    }
+    clip_area_backward<4, VertexSink>(sink.clip, sink.nc, sink.subj, sink.ns, c, sink);
+}
+
+// d(area of each destination cell)/dV times dL/da_u.
+__global__ void k_area_grad(const double *__restrict__ cells, int nv, const int32_t *__restrict__ vi,
+                            int64_t ncells, double kappa, const double *__restrict__ a_to, int64_t nf,
+                            const double *__restrict__ y, int64_t yc, int64_t yf,
+                            const double *__restrict__ gy, int64_t gc, int64_t gf,
+                            double *__restrict__ g) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncells) return;
+    double s = 0.0;
+    for (int64_t f = 0; f < nf; ++f) s += gy[t * gc + f * gf] * y[t * yc + f * yf];
+    double c = -kappa * s / a_to[t];
+    if (c == 0.0) return;
+    const int32_t *v = vi + (int64_t)nv * t;
+    ring_area_backward(cells + 3 * (int64_t)nv * t, nv, c, [&](int slot, const d3 &d) {
+        atomicAdd(g + 3 * (int64_t)v[slot], d.x);
+        atomicAdd(g + 3 * (int64_t)v[slot] + 1, d.y);
+        atomicAdd(g + 3 * (int64_t)v[slot] + 2, d.z);
+    });
+}
+
+__global__ void k_div_rows(const double *__restrict__ gy, int64_t gc, int64_t gf, int64_t n, int64_t nf,
+                           const double *__restrict__ a, double *__restrict__ w) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * nf) return;
+    int64_t f = t / n, i = t - f * n;
+    w[t] = a ? gy[i * gc + f * gf] / a[i] : gy[i * gc + f * gf];
+}
+
+void apply(const cr_regridder *R, bool transpose, int dtype, int64_t nf, const void *src, int64_t sc,
+           int64_t sf, void *dst, int64_t dc, int64_t df, bool normalize, hipStream_t s,
+           int force_variant);
+
+}  // namespace cr
+
+using namespace cr;
+
+extern "C" {
+
+cr_status cr_regrid_backward(const cr_regridder *R, const cr_grid *dst, const cr_grid *src,
+                             int32_t transpose, int32_t normalize, int64_t nfields, const double *x,
+                             int64_t xc, int64_t xf, const double *y, int64_t yc, int64_t yf,
+                             const double *gy, int64_t gc, int64_t gf, double *gx, int64_t gxc,
+                             int64_t gxf, double *g_dst_verts, double *g_src_verts, void *stream) {
+    return guard([&] {
+        CR_REQUIRE(R && nfields > 0 && x && gy, CR_INVALID_ARGUMENT, "cr_regrid_backward: bad arguments");
+        CR_REQUIRE(xc > 0 && gc > 0 && (nfields == 1 || (xf > 0 && gf > 0)), CR_INVALID_ARGUMENT,
+                   "cr_regrid_backward: bad strides");
+        const bool want_verts = g_dst_verts || g_src_verts;
+        hipStream_t s = as_stream(stream);
+        const Csr &F = R->fwd;
+        const int64_t n_to = transpose ? F.ncols : F.nrows, n_from = transpose ? F.nrows : F.ncols;
+        const double *a_to = transpose ? R->a_src.p : R->a_dst.p;
+        const int B = 256;
+        // field gradient: gx = A_dir^T (gy ./ a_to)
+        if (gx) {
+            CR_REQUIRE(gxc > 0 && (nfields == 1 || gxf > 0), CR_INVALID_ARGUMENT, "bad gx strides");
+            DevBuf<double> w((size_t)n_to * nfields, s);
+            hipLaunchKernelGGL(k_div_rows, dim3(grid_for(n_to * nfields, B)), dim3(B), 0, s, gy, gc, gf, n_to,
+                               nfields, normalize ? a_to : nullptr, w.p);
+            CR_HIP(hipGetLastError());
+            apply(R, !transpose, CR_F64, nfields, w.p, 1, n_to, gx, gxc, gxf, false, s, -1);
+            (void)n_from;
+        }
+        if (!want_verts) return;
+        CR_REQUIRE(R->area_scale > 0.0, CR_UNSUPPORTED,
+                   "vertex gradients need a regridder built from grids (cr_regridder_build)");
+        CR_REQUIRE(dst && src && dst->ncells == F.nrows && src->ncells == F.ncols, CR_DIMENSION_MISMATCH,
+                   "cr_regrid_backward: grids do not match the regridder");
+        const int clip_nv = R->stats.clip_with_dst ? dst->nv : src->nv;
+        CR_REQUIRE(clip_nv <= 4, CR_UNSUPPORTED, "vertex gradients need clip cells with at most 4 vertices");
+        if (normalize) CR_REQUIRE(y, CR_INVALID_ARGUMENT, "cr_regrid_backward: the forward output y is needed");
+        const double kappa = R->area_scale;
+        DevBuf<double> coef(F.nnz, s);
+        DevBuf<int32_t> rowid(F.nnz, s);
+        hipLaunchKernelGGL(k_pair_coef, dim3(grid_for(F.nrows, B)), dim3(B), 0, s, F.rowptr.p, F.col.p, F.nrows,
+                           (int)(transpose != 0), (int)(normalize != 0), kappa, a_to, nfields, x, xc, xf, gy,
+                           gc, gf, coef.p, rowid.p);
+        if (F.nnz)
+            hipLaunchKernelGGL(k_clip_grad, dim3(grid_for(F.nnz, B)), dim3(B), 0, s, rowid.p, F.col.p, coef.p,
+                               F.nnz, dst->cells.p, dst->nv, dst->vidx.p, src->cells.p, src->nv, src->vidx.p,
+                               R->stats.clip_with_dst, g_dst_verts, g_src_verts);
+        double *g_to = transpose ? g_src_verts : g_dst_verts;
+        const cr_grid *gto = transpose ? src : dst;
+        if (normalize && g_to)
+            hipLaunchKernelGGL(k_area_grad, dim3(grid_for(n_to, B)), dim3(B), 0, s, gto->cells.p, gto->nv,
+                               gto->vidx.p, n_to, kappa, a_to, nfields, y, yc, yf, gy, gc, gf, g_to);
+        CR_HIP(hipGetLastError());
+        CR_HIP(hipStreamSynchronize(s));
+    });
+}
+
+}  // extern "C"

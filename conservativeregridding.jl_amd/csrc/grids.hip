@@ -12,17 +12,24 @@ namespace cr {
 // (CellBasedGrid getcell, src/trees/grids.jl:71-84), index b*ni*nj + i + j*ni
 // (linear_to_cartesian_idx, src/trees/interfaces.jl:236-243).
 __global__ void k_points_to_cells(const double *__restrict__ pts, int64_t nblocks, int64_t ni,
-                                  int64_t nj, double *__restrict__ cells) {
+                                  int64_t nj, double *__restrict__ cells, int32_t *__restrict__ vidx) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t per = ni * nj;
     if (c >= nblocks * per) return;
     int64_t b = c / per, r = c - b * per;
     int64_t j = r / ni, i = r - j * ni;
     int64_t W = ni + 1;
-    const double *P = pts + 3 * b * W * (nj + 1);
+    int64_t base = b * W * (nj + 1);
     int64_t idx[4] = {i + j * W, i + 1 + j * W, i + 1 + (j + 1) * W, i + (j + 1) * W};
-    for (int k = 0; k < 4; ++k)
-        for (int d = 0; d < 3; ++d) cells[(c * 4 + k) * 3 + d] = P[idx[k] * 3 + d];
+    for (int k = 0; k < 4; ++k) {
+        for (int d = 0; d < 3; ++d) cells[(c * 4 + k) * 3 + d] = pts[(base + idx[k]) * 3 + d];
+        vidx[c * 4 + k] = (int32_t)(base + idx[k]);
+    }
+}
+
+__global__ void k_iota_i32(int32_t *__restrict__ x, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = (int32_t)i;
 }
 
 __global__ void k_healpix_cells(int64_t nside, int order, int step, int64_t first, int64_t count,
@@ -39,18 +46,23 @@ __global__ void k_healpix_cells(int64_t nside, int order, int step, int64_t firs
 }
 
 // Re-orient clockwise cells to counter-clockwise (reverse the ring) and count them.
-__global__ void k_orient_cells(double *__restrict__ cells, int64_t ncells, int nv,
-                               int *__restrict__ nflipped) {
+__global__ void k_orient_cells(double *__restrict__ cells, int32_t *__restrict__ vidx, int64_t ncells,
+                               int nv, int *__restrict__ nflipped) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncells) return;
     double *p = cells + 3 * nv * c;
     if (ring_area(p, nv) >= 0.0) return;
-    for (int a = 1, b = nv - 1; a < b; ++a, --b)
+    int32_t *vi = vidx + (int64_t)nv * c;
+    for (int a = 1, b = nv - 1; a < b; ++a, --b) {
         for (int d = 0; d < 3; ++d) {
             double t = p[3 * a + d];
             p[3 * a + d] = p[3 * b + d];
             p[3 * b + d] = t;
         }
+        int32_t t = vi[a];
+        vi[a] = vi[b];
+        vi[b] = t;
+    }
     atomicAdd(nflipped, 1);
 }
 
@@ -65,7 +77,7 @@ void orient_grid(cr_grid *g, hipStream_t s) {
     DevBuf<int> cnt(1);
     CR_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int), s));
     hipLaunchKernelGGL(k_orient_cells, dim3(grid_for(g->ncells, 256)), dim3(256), 0, s, g->cells.p,
-                       g->ncells, g->nv, cnt.p);
+                       g->vidx.p, g->ncells, g->nv, cnt.p);
     CR_HIP(hipGetLastError());
     int h = 0;
     CR_HIP(hipMemcpyAsync(&h, cnt.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -198,6 +210,10 @@ cr_status cr_grid_from_cells(cr_grid **grid, const double *cells, int64_t ncells
             g->convex = convex ? 1 : 0;
             g->cells.alloc((size_t)ncells * nv * 3);
             CR_HIP(hipMemcpyAsync(g->cells.p, cells, g->cells.bytes(), hipMemcpyHostToDevice, s));
+            g->nverts = ncells * nv;
+            g->vidx.alloc((size_t)ncells * nv);
+            hipLaunchKernelGGL(k_iota_i32, dim3(grid_for(g->nverts, 256)), dim3(256), 0, s, g->vidx.p,
+                               g->nverts);
             orient_grid(g, s);
         } catch (...) {
             delete g;
@@ -224,8 +240,10 @@ cr_status cr_grid_from_points(cr_grid **grid, const double *pts, int64_t nblocks
             g->nv = 4;
             g->convex = 1;
             g->cells.alloc((size_t)g->ncells * 4 * 3);
+            g->nverts = npts;
+            g->vidx.alloc((size_t)g->ncells * 4);
             hipLaunchKernelGGL(k_points_to_cells, dim3(grid_for(g->ncells, 256)), dim3(256), 0, s,
-                               dpts.p, nblocks, ni, nj, g->cells.p);
+                               dpts.p, nblocks, ni, nj, g->cells.p, g->vidx.p);
             CR_HIP(hipGetLastError());
             orient_grid(g, s);
         } catch (...) {
@@ -256,6 +274,9 @@ cr_status cr_grid_healpix(cr_grid **grid, int64_t nside, int32_t order, int32_t 
             int64_t nthr = g->ncells * g->nv;
             hipLaunchKernelGGL(k_healpix_cells, dim3(grid_for(nthr, 256)), dim3(256), 0, s, nside, order,
                                step, int64_t(0), g->ncells, g->cells.p);
+            g->nverts = nthr;
+            g->vidx.alloc((size_t)nthr);
+            hipLaunchKernelGGL(k_iota_i32, dim3(grid_for(nthr, 256)), dim3(256), 0, s, g->vidx.p, nthr);
             CR_HIP(hipGetLastError());
             CR_HIP(hipStreamSynchronize(s));
         } catch (...) {
@@ -272,6 +293,13 @@ cr_status cr_grid_info(const cr_grid *grid, int64_t *ncells, int32_t *nv, int32_
         if (ncells) *ncells = grid->ncells;
         if (nv) *nv = grid->nv;
         if (convex) *convex = grid->convex;
+    });
+}
+
+cr_status cr_grid_vertex_count(const cr_grid *grid, int64_t *nverts) {
+    return guard([&] {
+        CR_REQUIRE(grid && nverts, CR_INVALID_ARGUMENT, "null argument");
+        *nverts = grid->nverts;
     });
 }
 

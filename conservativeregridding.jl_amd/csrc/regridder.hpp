@@ -33,6 +33,9 @@ struct cr_regridder {
     cr::Csr bwd;  // A^T: n_src x n_dst
     cr::DevBuf<double> a_dst, a_src;
     cr_build_stats stats{};
+    // stored areas = area_scale * unit-sphere areas (radius^2, divided by max(A)
+    // after normalize!); 0 when built from arrays (no geometry: no vertex gradients)
+    double area_scale = 0.0;
 };
 
 namespace cr {

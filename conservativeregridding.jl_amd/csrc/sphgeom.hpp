@@ -194,20 +194,26 @@ CR_HD double ring_area(const double *xyz, int n) {
 // crossing x = y/|y|, y = +-(n1 x n2), n = p x q, to the input vertices.
 // ============================================================================
 
-// label: bits 30-31 type, low bits indices
+// label: bits 28-29 type, bits 8-27 and 0-7 indices (a subject vertex uses
+// bits 0-27); -1 (sign bit) marks a vertex without provenance
 constexpr int32_t kLabS = 0, kLabC = 1, kLabXF = 2, kLabXE = 3;
-CR_HD int32_t lab_make(int32_t type, int32_t a, int32_t b) { return (type << 30) | (a << 8) | b; }
-CR_HD int32_t lab_type(int32_t l) { return (l >> 30) & 3; }
-CR_HD int32_t lab_hi(int32_t l) { return (l >> 8) & 0x3fffff; }
+constexpr int32_t kLabIndexMask = 0x0fffffff;
+CR_HD int32_t lab_make(int32_t type, int32_t a, int32_t b) { return (type << 28) | (a << 8) | b; }
+CR_HD int32_t lab_type(int32_t l) { return (l >> 28) & 3; }
+CR_HD int32_t lab_hi(int32_t l) { return (l >> 8) & 0xfffff; }
 CR_HD int32_t lab_lo(int32_t l) { return l & 0xff; }
 
 // line ids: subject edge e -> e, clip edge k -> kClipLine | k
-constexpr int32_t kClipLine = 1 << 29;
+constexpr int32_t kClipLine = 1 << 27;
 
 template <int NC>
 CR_HD void lab_lines(int32_t l, int ns, int32_t &l1, int32_t &l2) {
     int32_t t = lab_type(l), a = lab_hi(l), b = lab_lo(l);
-    if (t == kLabS) { l1 = (a + ns - 1) % ns; l2 = a; }
+    if (t == kLabS) {
+        int32_t e = l & kLabIndexMask;
+        l1 = (e + ns - 1) % ns;
+        l2 = e;
+    }
     else if (t == kLabC) { l1 = kClipLine | ((b + NC - 1) % NC); l2 = kClipLine | b; }
     else if (t == kLabXF) { l1 = a; l2 = kClipLine | b; }
     else { l1 = kClipLine | a; l2 = kClipLine | b; }

@@ -113,6 +113,9 @@ class Grid:
         self.ncells, self.nv, self.convex = n.value, nv.value, bool(cv.value)
         self.shape = tuple(shape)
         self.kind = kind
+        nvert = ctypes.c_int64()
+        L.check(L.lib.cr_grid_vertex_count(self._h, ctypes.byref(nvert)))
+        self.nverts = nvert.value  # caller-visible vertices (gradients are indexed by these)
 
     # -- constructors ---------------------------------------------------------
     @classmethod

@@ -420,3 +420,36 @@ def regrid_tune(dst_field, R: Regridder, src_field, *, dims: int = 1):
                                  dst_field.data_ptr(), dst_field.stride(dim), df, ctypes.byref(chosen),
                                  ctypes.byref(ms), torch.cuda.current_stream().cuda_stream), "regrid_tune")
     return chosen.value, ms.value
+
+
+def regrid_backward(R: Regridder, dst_grid, src_grid, x, y, gy, *, normalize: bool = True,
+                    field_grad: bool = True, vertex_grad: bool = True):
+    """Reverse mode of ``y = regrid(R, x)`` (FP64 device tensors, cells along
+    dim 0, fields along dim 1 if 2-D).  Returns ``(gx, g_dst, g_src)``: the
+    gradient w.r.t. x and w.r.t. the caller vertices of the two grids R was
+    built from (``(grid.nverts, 3)`` tensors; tangential).  For ``transpose(R)``
+    the roles of the grids follow the regridder's direction."""
+    import torch
+    for t in (x, gy) + ((y,) if normalize else ()):
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64):
+            raise MethodError("regrid_backward needs FP64 CUDA tensors")
+    nf = 1 if x.ndim == 1 else x.shape[1]
+
+    def st(t):
+        return (t.stride(0), t.stride(1) if t.ndim > 1 else 1)
+
+    gx = torch.empty_like(x, memory_format=torch.contiguous_format) if field_grad else None
+    g_dst = g_src = None
+    if vertex_grad:
+        g_dst = torch.zeros(dst_grid.nverts, 3, dtype=torch.float64, device=x.device)
+        g_src = torch.zeros(src_grid.nverts, 3, dtype=torch.float64, device=x.device)
+    (xc, xf), (gc, gf) = st(x), st(gy)
+    yc, yf = st(y) if y is not None else (1, 1)
+    gxc, gxf = st(gx) if gx is not None else (1, 1)
+    L.check(L.lib.cr_regrid_backward(
+        R._core.h, dst_grid._h, src_grid._h, 1 if R._t else 0, 1 if normalize else 0, nf,
+        x.data_ptr(), xc, xf, y.data_ptr() if y is not None else None, yc, yf, gy.data_ptr(), gc, gf,
+        gx.data_ptr() if gx is not None else None, gxc, gxf,
+        g_dst.data_ptr() if g_dst is not None else None, g_src.data_ptr() if g_src is not None else None,
+        torch.cuda.current_stream().cuda_stream), "regrid_backward")
+    return gx, g_dst, g_src

@@ -122,6 +122,10 @@ cr_status cr_grid_from_points(cr_grid **grid, const double *pts, int64_t nblocks
 cr_status cr_grid_healpix(cr_grid **grid, int64_t nside, int32_t order, int32_t step,
                           void *stream);
 cr_status cr_grid_info(const cr_grid *grid, int64_t *ncells, int32_t *nv, int32_t *convex);
+/* Number of caller-visible vertices: points of a point grid ((ni+1)(nj+1) per
+ * block), ncells*nv for a polygon table or a HEALPix grid.  Vertex gradients
+ * (cr_regrid_backward) are indexed by these. */
+cr_status cr_grid_vertex_count(const cr_grid *grid, int64_t *nverts);
 /* Download the (re-oriented) polygon table, ncells*nv*3 doubles. */
 cr_status cr_grid_copy_cells(const cr_grid *grid, double *cells);
 /* Cell areas on the unit sphere, computed on the device (areas(), regridder.jl:165-178). */
@@ -165,6 +169,25 @@ cr_status cr_regrid(const cr_regridder *R, int32_t transpose, int32_t dtype, int
                     const void *src, int64_t src_cell_stride, int64_t src_field_stride,
                     void *dst, int64_t dst_cell_stride, int64_t dst_field_stride,
                     int32_t normalize, void *stream);
+
+/*
+ * Reverse mode of regrid!(y, R, x) (transpose=1: of regrid!(y, transpose(R), x)),
+ * the kernel behind a ChainRulesCore.rrule for regrid! (no reference
+ * counterpart: the reference has no AD).  FP64 device arrays, strides as in
+ * cr_regrid.  Given the forward input x, output y and the gradient gy of a loss
+ * w.r.t. y:
+ *   gx (overwritten, may be NULL)  = dL/dx = A^T (gy ./ a_to)
+ *   g_dst_verts, g_src_verts (3 doubles per caller vertex of the grids, see
+ *   cr_grid_vertex_count; ACCUMULATED with FP64 atomics, may be NULL) = dL/dV
+ *   through every intersection area and the destination cell areas.
+ * dst and src must be the grids R was built from (vertex gradients are not
+ * available for cr_regridder_from_csr regridders).  y is needed when normalize=1.
+ */
+cr_status cr_regrid_backward(const cr_regridder *R, const cr_grid *dst, const cr_grid *src,
+                             int32_t transpose, int32_t normalize, int64_t nfields, const double *x,
+                             int64_t xc, int64_t xf, const double *y, int64_t yc, int64_t yf,
+                             const double *gy, int64_t gc, int64_t gf, double *gx, int64_t gxc,
+                             int64_t gxf, double *g_dst_verts, double *g_src_verts, void *stream);
 
 /*
  * Autotuner for the FP32 cells-fastest batched apply (no reference
