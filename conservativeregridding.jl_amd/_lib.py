@@ -82,6 +82,8 @@ SIGNATURES = {
                                    c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp]),
     "cr_regrid_tune": (c_i32, [c_vp, c_i32, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, _i32p,
                                ctypes.POINTER(ctypes.c_float), c_vp]),
+    "cr_regrid_variant_count": (c_i32, [c_i32]),
+    "cr_regrid_set_variant": (c_i32, [c_vp, c_i32, c_i32, c_i32]),
 }
 
 

@@ -7,6 +7,9 @@
 // flop per byte), so the apply is bound by HBM bandwidth: no MFMA.
 //
 // Kernels
+//   k_spmm_tiles<T,F>   (tiles.hip) staged merge-path SpMM over row tiles:
+//                       distinct source cells of a tile gathered once per field
+//                       into LDS, non-zeros split evenly over threads
 //   k_apply_rows<T,FB>  one thread per row; blockIdx.y = block of FB fields,
 //                       the slowest launch dimension; XCD-contiguous row tiles
 //   k_apply_long<T,FB>  rows with more than kLongRow non-zeros, one wave each
@@ -251,26 +254,63 @@ static void launch_rows(const Csr &A, const T *x, int64_t sc, int64_t sf, T *y, 
                            inv);
 }
 
-// Field-block widths the autotuner tries for FP32 fields (index = variant).
-constexpr int kFieldBlocks[] = {1, 2, 4, 8, 16};
-constexpr int kNumVariants = sizeof(kFieldBlocks) / sizeof(kFieldBlocks[0]);
-constexpr int kDefaultF32Variant = 3;  // FB = 8
+// Staged tile kernel (tiles.hip) followed by the rows it leaves to the long-row
+// kernel.  scale: 1/area for FP32 fields, area for FP64 fields, or nullptr.
+template <class T>
+void launch_tiles(const TilePlan &P, int F, int nchunks, const T *x, int64_t sc, int64_t sf, T *y,
+                  int64_t dc, int64_t df, int64_t nf, const double *scale, hipStream_t s);
+
+template <class T>
+static void launch_tiled(const Csr &A, int F, int nchunks, const T *x, int64_t sc, int64_t sf, T *y,
+                         int64_t dc, int64_t df, int64_t nf, const double *area, const double *inv,
+                         hipStream_t s) {
+    const TilePlan &P = tile_plan(A, s);
+    constexpr bool f32 = sizeof(T) == 4;
+    launch_tiles<T>(P, F, nchunks, x, sc, sf, y, dc, df, nf, f32 ? inv : area, s);
+    if (P.nhuge) {
+        constexpr int FB = f32 ? 8 : 1;
+        const int64_t gy = (nf + FB - 1) / FB;
+        CR_REQUIRE(gy <= 65535, CR_UNSUPPORTED, "too many fields for one launch");
+        hipLaunchKernelGGL((k_apply_long<T, FB>), dim3((unsigned)P.nhuge, (unsigned)gy), dim3(64), 0, s,
+                           P.huge.p, A.rowptr.p, A.col.p, A.val.p, x, sc, sf, y, dc, df, nf, area, inv);
+    }
+}
+
+// FP32 kernel variants the autotuner tries (index = variant): the row kernel
+// with field-block widths 1..16, then the staged tile kernel with 2, 4 or 8
+// fields per LDS stage.
+struct F32Variant {
+    bool tiled;
+    int width;  // FB (row kernel) or F (tile kernel)
+};
+constexpr F32Variant kF32Variants[] = {{false, 1}, {false, 2}, {false, 4}, {false, 8}, {false, 16},
+                                       {true, 2},  {true, 4},  {true, 8}};
+constexpr int kNumVariants = sizeof(kF32Variants) / sizeof(kF32Variants[0]);
+constexpr int kDefaultF32Variant = 6;  // tiles, 4 fields per stage
 
 static void launch_f32_variant(int v, const Csr &A, const float *x, int64_t sc, int64_t sf, float *y,
                                int64_t dc, int64_t df, int64_t nf, const double *inv, hipStream_t s) {
-    switch (v) {
-        case 0: launch_rows<float, 1>(A, x, sc, sf, y, dc, df, nf, nullptr, inv, s); break;
-        case 1: launch_rows<float, 2>(A, x, sc, sf, y, dc, df, nf, nullptr, inv, s); break;
-        case 2: launch_rows<float, 4>(A, x, sc, sf, y, dc, df, nf, nullptr, inv, s); break;
-        case 3: launch_rows<float, 8>(A, x, sc, sf, y, dc, df, nf, nullptr, inv, s); break;
+    const F32Variant &V = kF32Variants[v];
+    if (V.tiled) {
+        launch_tiled<float>(A, V.width, 0, x, sc, sf, y, dc, df, nf, nullptr, inv, s);
+        return;
+    }
+    switch (V.width) {
+        case 1: launch_rows<float, 1>(A, x, sc, sf, y, dc, df, nf, nullptr, inv, s); break;
+        case 2: launch_rows<float, 2>(A, x, sc, sf, y, dc, df, nf, nullptr, inv, s); break;
+        case 4: launch_rows<float, 4>(A, x, sc, sf, y, dc, df, nf, nullptr, inv, s); break;
+        case 8: launch_rows<float, 8>(A, x, sc, sf, y, dc, df, nf, nullptr, inv, s); break;
         default: launch_rows<float, 16>(A, x, sc, sf, y, dc, df, nf, nullptr, inv, s); break;
     }
 }
 
-// Kernel choice: FP64 -> row kernel (1 field per block for a single field, 8
-// otherwise); FP32 with unit field strides and many fields -> fields-fastest
-// kernel; other FP32 layouts -> the row kernel with the autotuned (or default)
-// field-block width.
+constexpr int kNumF64Variants = 2;  // 0 row kernel, 1 tile kernel
+constexpr int kDefaultF64Variant = 1;
+
+// Kernel choice: FP64 -> the tile kernel (1 or 2 fields per stage) unless the
+// row kernel is pinned; FP32 with unit field strides and many fields ->
+// fields-fastest kernel; other FP32 layouts -> the autotuned (or pinned, or
+// default: tiles, 4 fields per stage) variant.
 void apply(const cr_regridder *R, bool transpose, int dtype, int64_t nf, const void *src,
            int64_t sc, int64_t sf, void *dst, int64_t dc, int64_t df, bool normalize, hipStream_t s,
            int force_variant = -1) {
@@ -281,7 +321,9 @@ void apply(const cr_regridder *R, bool transpose, int dtype, int64_t nf, const v
         const double *x = (const double *)src;
         double *y = (double *)dst;
         const double *ar = normalize ? area : nullptr;
-        if (nf == 1) launch_rows<double, 1>(A, x, sc, sf, y, dc, df, nf, ar, nullptr, s);
+        const int v = A.f64_variant >= 0 ? A.f64_variant : kDefaultF64Variant;
+        if (v == 1) launch_tiled<double>(A, nf >= 2 ? 2 : 1, 0, x, sc, sf, y, dc, df, nf, ar, nullptr, s);
+        else if (nf == 1) launch_rows<double, 1>(A, x, sc, sf, y, dc, df, nf, ar, nullptr, s);
         else launch_rows<double, 8>(A, x, sc, sf, y, dc, df, nf, ar, nullptr, s);
     } else if (sf == 1 && df == 1 && nf >= 64 && force_variant < 0) {
         hipLaunchKernelGGL(k_apply_fieldsfast, dim3(grid_for(A.nrows, 4)), dim3(256), 0, s, A.rowptr.p,
@@ -361,6 +403,22 @@ cr_status cr_regrid_tune(cr_regridder *R, int32_t transpose, int64_t nfields, co
         A.tuned_variant = best;
         if (chosen) *chosen = best;
         if (best_ms) *best_ms = best_t;
+    });
+}
+
+int32_t cr_regrid_variant_count(int32_t dtype) {
+    return dtype == CR_F32 ? kNumVariants : dtype == CR_F64 ? kNumF64Variants : 0;
+}
+
+cr_status cr_regrid_set_variant(cr_regridder *R, int32_t transpose, int32_t dtype, int32_t variant) {
+    return guard([&] {
+        CR_REQUIRE(R, CR_INVALID_ARGUMENT, "cr_regrid_set_variant: null regridder");
+        CR_REQUIRE(dtype == CR_F64 || dtype == CR_F32, CR_UNSUPPORTED, "cr_regrid_set_variant: dtype must be F64 or F32");
+        const int n = dtype == CR_F32 ? kNumVariants : kNumF64Variants;
+        CR_REQUIRE(variant >= -1 && variant < n, CR_INVALID_ARGUMENT, "cr_regrid_set_variant: no such variant");
+        Csr &A = transpose ? R->bwd : R->fwd;
+        if (dtype == CR_F32) A.tuned_variant = variant;
+        else A.f64_variant = variant;
     });
 }
 

@@ -5,9 +5,10 @@
 //   treeify + dual-tree DFS over spherical caps (MultithreadedDualDepthFirstSearch.jl)
 //     -> k_cell_bounds: FP64 area + bounding cap of every cell
 //        Morton sort of source caps + implicit 8-ary tree of caps (k_build_level)
-//        k_traverse<false/true>: one thread per destination cell walks the tree
-//        stacklessly; count pass + scan + fill pass emit candidates grouped by
-//        destination row in a deterministic order
+//        k_search<false/true>: one wave per Morton cluster of 64 destination
+//        cells walks the tree with the cluster's cap, then every lane tests its
+//        cell against the cluster's leaf list; count pass + scan + fill pass
+//        emit candidates grouped by destination row in a deterministic order
 //   compute_intersection_areas with ConvexConvexSutherlandHodgman + GO.area
 //     -> k_clip<4>: one thread per candidate, streaming FP64 clip + area
 //   keep iff area > 0 (here > threshold_rel * min(a_src, a_dst)) and
@@ -61,37 +62,154 @@ __global__ void k_build_level(const float4 *__restrict__ child, int64_t nchild,
     parent[p] = parent_cap(child, c0, c1);
 }
 
-// One thread per destination cell walks the source tree.  FILL=false counts
-// candidates; FILL=true writes them at the scanned offsets (never past the
-// row's segment) in the same deterministic order.
+// Candidate search, one wave per 64 destination cells taken in Morton order of
+// their caps (a compact cluster).  The wave bounds its 64 caps by one cap,
+// walks the source tree level by level with that cap (lanes test the children
+// of the current frontier side by side; ballot + prefix count compact the
+// survivors into LDS), and ends with the list of source leaves that touch the
+// cluster.  Each lane then tests its own cap against that list (LDS broadcast
+// reads, no divergence).  A cluster whose frontier or leaf list does not fit
+// the LDS lists falls back to a per-lane stackless walk (tree_query).  Every
+// leaf that touches a cell's cap touches the cluster cap (it contains the
+// cell's cap) and all of its ancestors, so both routes emit exactly the
+// leaves whose caps touch the cell's cap, in a fixed order.  FILL=false
+// counts; FILL=true writes at the scanned offsets (never past the cell's
+// segment).
+constexpr int kSearchLeaves = 512;
+constexpr int kSearchFront = 256;
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, 64));
+    return v;
+}
+
 template <bool FILL>
-__global__ void __launch_bounds__(256) k_traverse(const float4 *__restrict__ qcap, int64_t nq,
-                                                  const float4 *__restrict__ nodes, Levels lv,
-                                                  const int32_t *__restrict__ perm,
-                                                  int64_t *__restrict__ count_or_offset,
-                                                  int32_t *__restrict__ cand_src,
-                                                  int32_t *__restrict__ cand_dst) {
-    int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= nq) return;
+__global__ void __launch_bounds__(64) k_search(const float4 *__restrict__ qcap,
+                                               const int32_t *__restrict__ qorder, int64_t nq,
+                                               const float4 *__restrict__ nodes, Levels lv,
+                                               const int32_t *__restrict__ perm,
+                                               int64_t *__restrict__ count_or_offset,
+                                               int32_t *__restrict__ cand_src,
+                                               int32_t *__restrict__ cand_dst) {
+    __shared__ float4 lcap[kSearchLeaves];
+    __shared__ int32_t lid[kSearchLeaves];
+    __shared__ int32_t front[2][kSearchFront];
+    const int lane = threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+    const bool active = i < nq;
+    const int64_t d = active ? qorder[i] : 0;
+    const float4 c = active ? qcap[d] : make_float4(0.f, 0.f, 0.f, 0.f);
+
+    // cluster cap: centre = normalised sum of centres, radius = max(|c - m| + r)
+    const float sx = wave_sum(active ? c.x : 0.f), sy = wave_sum(active ? c.y : 0.f),
+                sz = wave_sum(active ? c.z : 0.f);
+    const float L = sqrtf(sx * sx + sy * sy + sz * sz);
+    const bool wide = __ballot(active && c.w >= 2.0f) != 0ull;
+    bool overflow = wide || L < 1e-3f;
+    int nleaf = 0;
+    if (!overflow) {
+        const float mx = sx / L, my = sy / L, mz = sz / L;
+        float dd = 0.f;
+        if (active) {
+            const float dx = c.x - mx, dy = c.y - my, dz = c.z - mz;
+            dd = sqrtf(dx * dx + dy * dy + dz * dz) + c.w;
+        }
+        const float rad = wave_max(dd) * (1.0f + 1e-5f) + 1e-5f;
+        overflow = rad >= 1.4f;
+        const float4 wc = make_float4(mx, my, mz, rad);
+        if (!overflow) {
+            if (lv.top == 0) {  // a single leaf
+                if (lane == 0) {
+                    lcap[0] = nodes[0];
+                    lid[0] = perm[0];
+                }
+                nleaf = 1;
+            } else {
+                if (lane == 0) front[0][0] = 0;
+                __syncthreads();
+                int cur = 0, nf = 1;
+                for (int lev = lv.top; lev >= 1; --lev) {
+                    const int64_t csize = lv.size[lev - 1], coff = lv.off[lev - 1];
+                    const int tot = nf * kTreeFanout;
+                    int nn = 0;
+                    for (int base = 0; base < tot; base += 64) {
+                        const int t = base + lane;
+                        bool hit = false;
+                        int64_t child = 0;
+                        float4 cc = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if (t < tot) {
+                            child = (int64_t)front[cur][t >> 3] * kTreeFanout + (t & 7);
+                            if (child < csize) {
+                                cc = nodes[coff + child];
+                                hit = caps_overlap(wc, cc);
+                            }
+                        }
+                        const uint64_t m = __ballot(hit);
+                        const int pos = nn + (int)__popcll(m & ((1ull << lane) - 1ull));
+                        if (hit) {
+                            if (lev > 1) {
+                                if (pos < kSearchFront) front[cur ^ 1][pos] = (int32_t)child;
+                            } else if (pos < kSearchLeaves) {
+                                lcap[pos] = cc;
+                                lid[pos] = perm[child];
+                            }
+                        }
+                        nn += (int)__popcll(m);
+                    }
+                    __syncthreads();
+                    if (nn > (lev > 1 ? kSearchFront : kSearchLeaves)) {
+                        overflow = true;
+                        break;
+                    }
+                    cur ^= 1;
+                    nf = nn;
+                }
+                nleaf = nf;
+            }
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    if (overflow) {
+        if (FILL) {
+            int64_t pos = count_or_offset[d];
+            const int64_t end = count_or_offset[d + 1];
+            tree_query(c, nodes, lv, [&](int64_t leaf) {
+                if (pos < end) {
+                    cand_src[pos] = perm[leaf];
+                    cand_dst[pos] = (int32_t)d;
+                    ++pos;
+                }
+            });
+        } else {
+            int64_t cnt = 0;
+            tree_query(c, nodes, lv, [&](int64_t) { ++cnt; });
+            count_or_offset[d] = cnt;
+        }
+        return;
+    }
     if (FILL) {
         int64_t pos = count_or_offset[d];
         const int64_t end = count_or_offset[d + 1];
-        tree_query(qcap[d], nodes, lv, [&](int64_t i) {
-            if (pos < end) {
-                cand_src[pos] = perm[i];
+        for (int j = 0; j < nleaf; ++j)
+            if (caps_overlap(c, lcap[j]) && pos < end) {
+                cand_src[pos] = lid[j];
                 cand_dst[pos] = (int32_t)d;
                 ++pos;
             }
-        });
     } else {
         int64_t cnt = 0;
-        tree_query(qcap[d], nodes, lv, [&](int64_t) { ++cnt; });
+        for (int j = 0; j < nleaf; ++j) cnt += caps_overlap(c, lcap[j]);
         count_or_offset[d] = cnt;
     }
 }
 
-// One thread per candidate: FP64 area of (subject ∩ clip) via the streaming
-// clip pipeline; stores the area if kept, else 0.
 template <int NC>
 __global__ void __launch_bounds__(256) k_clip(const int32_t *__restrict__ cand_dst,
                                               const int32_t *__restrict__ cand_src, int64_t ncand,
@@ -349,13 +467,19 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     }
     tm.mark();  // 2
 
-    // 3. candidate search: count, scan, fill
+    // 3. candidate search: destination cells in Morton order, count, scan, fill
     DevBuf<int64_t> cand_off(nd + 1, s);
+    DevBuf<int32_t> qorder(nd, s);
+    const unsigned qblocks = grid_for(nd, 64);
     {
+        DevBuf<uint32_t> key(nd, s), key_sorted(nd, s);
+        DevBuf<int32_t> idx(nd, s);
+        hipLaunchKernelGGL(k_morton, dim3(grid_for(nd, B)), dim3(B), 0, s, cap_dst.p, nd, key.p, idx.p);
+        sort_pairs(T, key.p, key_sorted.p, idx.p, qorder.p, nd, 30, s);
         DevBuf<int64_t> cand_cnt(nd + 1, s);
         CR_HIP(hipMemsetAsync(cand_cnt.p + nd, 0, sizeof(int64_t), s));
-        hipLaunchKernelGGL(k_traverse<false>, dim3(grid_for(nd, B)), dim3(B), 0, s, cap_dst.p, nd,
-                           nodes.p, lv, perm.p, cand_cnt.p, (int32_t *)nullptr, (int32_t *)nullptr);
+        hipLaunchKernelGGL(k_search<false>, dim3(qblocks), dim3(64), 0, s, cap_dst.p, qorder.p, nd, nodes.p,
+                           lv, perm.p, cand_cnt.p, (int32_t *)nullptr, (int32_t *)nullptr);
         CR_HIP(hipGetLastError());
         exclusive_sum(T, cand_cnt.p, cand_off.p, nd + 1, s);
     }
@@ -369,8 +493,8 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
         CR_HIP(hipMemsetAsync(cand_dst.p, 0, cand_dst.bytes(), s));
     }
     if (ncand)
-        hipLaunchKernelGGL(k_traverse<true>, dim3(grid_for(nd, B)), dim3(B), 0, s, cap_dst.p, nd,
-                           nodes.p, lv, perm.p, cand_off.p, cand_src.p, cand_dst.p);
+        hipLaunchKernelGGL(k_search<true>, dim3(qblocks), dim3(64), 0, s, cap_dst.p, qorder.p, nd, nodes.p,
+                           lv, perm.p, cand_off.p, cand_src.p, cand_dst.p);
     CR_HIP(hipGetLastError());
     tm.mark();  // 3
 
@@ -500,6 +624,8 @@ void normalize_regridder(cr_regridder *R, hipStream_t s) {
     hipLaunchKernelGGL(k_scale, dim3(grid_for(F.ncols, B)), dim3(B), 0, s, R->a_src.p, F.ncols, 1.0, m);
     CR_HIP(hipGetLastError());
     R->area_scale /= m;
+    drop_tile_plan(F);
+    drop_tile_plan(R->bwd);
     finalize_regridder(R, s);
     CR_HIP(hipStreamSynchronize(s));
 }

@@ -10,9 +10,26 @@
 //   counterpart: cr_regrid takes strides.
 #pragma once
 
+#include <memory>
+#include <mutex>
+
 #include "common.hpp"
+#include "tileplan.hpp"
 
 namespace cr {
+
+struct TilePlan {
+    int64_t ntiles = 0;
+    DevBuf<TileDesc> tiles;
+    DevBuf<uint32_t> key;    // per slot: local column (low 16 bits) | segment id (high 16)
+    DevBuf<float> val32;     // per slot: A value rounded to FP32 (0 in padding slots)
+    DevBuf<double> val64;    // per slot: A value
+    DevBuf<int32_t> cols;    // distinct columns of each tile, ascending
+    DevBuf<uint16_t> rseg;   // per tile, nrows + 1 entries: first segment of each row
+    DevBuf<int32_t> huge;    // rows too long for a tile (done by the long-row kernel)
+    int64_t nhuge = 0;
+    int64_t nslots = 0;
+};
 
 struct Csr {
     int64_t nrows = 0, ncols = 0, nnz = 0;
@@ -23,8 +40,17 @@ struct Csr {
     DevBuf<int32_t> long_rows; // rows with more than kLongRow non-zeros (own kernel)
     int64_t nlong = 0;
     int64_t max_row_nnz = 0;
-    int tuned_variant = -1;    // FP32 field-block width chosen by cr_regrid_tune; -1: default
+    int tuned_variant = -1;    // FP32 kernel variant (cr_regrid_tune / set_variant); -1: default
+    int f64_variant = -1;      // FP64 kernel variant; -1: default
+    // staged-apply plan, made on first use (guarded: handles are shared by callers)
+    mutable std::shared_ptr<TilePlan> tiles;
+    mutable std::mutex tiles_mu;
 };
+
+// The tile plan of A, built on the first call (synchronises stream s once).
+const TilePlan &tile_plan(const Csr &A, hipStream_t s);
+// Forget the plan (after A's values change).
+void drop_tile_plan(const Csr &A);
 
 }  // namespace cr
 

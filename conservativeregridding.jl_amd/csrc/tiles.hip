@@ -1,0 +1,285 @@
+// tiles.hip -- staged batched apply: y[:, f] = A x[:, f] for many fields f.
+//
+// Reference: the batched regrid! is a serial loop of 1-D mul! calls over the
+// non-spatial indices (NDSliceLoop, src/regridder/regrid.jl:303-318).  Here one
+// launch does every field, organised so that each byte of the fields crosses
+// HBM about once:
+//
+//   * rows are cut into tiles (contiguous rows, at most kTileRows rows,
+//     kTileCols distinct columns and kTileThreads * kTileEnt non-zeros);
+//   * a tile's non-zeros are dealt out evenly to the threads of its workgroup
+//     (merge-path split: thread t takes entries [t*E, (t+1)*E), cutting rows
+//     anywhere), so long and short rows cost the same per thread;
+//   * each thread keeps its E entries (packed local column + segment id, and
+//     the value) in registers for all fields of the launch;
+//   * per field, the tile's distinct columns of x are loaded once (ascending
+//     column order: runs of neighbouring cells, coalesced) into LDS, every
+//     thread sums its row segments from LDS, and a second pass adds the
+//     segments of each row in order and stores the rows coalesced.
+//
+// Launch: one workgroup per (tile, field chunk).  Workgroups go round-robin to
+// the 8 XCDs; workgroup b works on tile (b % 8) * tiles_per_xcd + (b / 8) /
+// nchunks, so every XCD owns a contiguous range of tiles (shared source cells
+// stay in its L2) and the chunks of one tile run side by side (the tile's
+// entries are read from HBM once).
+//
+// Numerics match the row kernels: FP32 fields use the FP64 A rounded to FP32,
+// FP32 products and Kahan-compensated segment sums, segments added in FP64 and
+// scaled by the FP64 reciprocal area; FP64 fields use fused multiply-adds and a
+// true division by the area.
+#include <algorithm>
+#include <vector>
+
+#include "regridder.hpp"
+#include "tileplan.hpp"
+
+namespace cr {
+
+
+template <class T>
+struct SegAcc;
+
+template <>
+struct SegAcc<double> {
+    double s;
+    __device__ void init() { s = 0.0; }
+    __device__ void add(double v, double x) { s = fma(v, x, s); }
+    __device__ double value() const { return s; }
+};
+
+template <>
+struct SegAcc<float> {
+    float s, c;  // Kahan: the compensated sum is s - c
+    __device__ void init() {
+        s = 0.f;
+        c = 0.f;
+    }
+    __device__ void add(float v, float x) {
+        float yk = v * x - c;
+        float t = s + yk;
+        c = (t - s) - yk;
+        s = t;
+    }
+    __device__ float value() const { return (float)((double)s - (double)c); }
+};
+
+// scale = 1/area (FP32 fields) or area (FP64 fields), nullptr: no normalisation
+__device__ __forceinline__ float row_finish(double acc, double scale, bool has, float *) {
+    return (float)(has ? acc * scale : acc);
+}
+__device__ __forceinline__ double row_finish(double acc, double scale, bool has, double *) {
+    return has ? acc / scale : acc;
+}
+
+template <class T, int F>
+__global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
+    const TileDesc *__restrict__ tiles, int64_t ntiles, int64_t tiles_per_xcd, int nchunks,
+    int64_t chunk, const uint32_t *__restrict__ keys, const T *__restrict__ vals,
+    const int32_t *__restrict__ cols, const uint16_t *__restrict__ rseg, const T *__restrict__ x,
+    int64_t sc, int64_t sf, T *__restrict__ y, int64_t dc, int64_t df, int64_t nf,
+    const double *__restrict__ scale) {
+    __shared__ T xs[F][kTileCols];
+    __shared__ T seg[F][kTileSegs + 1];
+    const int64_t b = blockIdx.x;
+    const int64_t local = b >> 3;
+    const int64_t tile = (b & 7) * tiles_per_xcd + local / nchunks;
+    if (tile >= ntiles) return;
+    const int64_t fb = (local % nchunks) * chunk;
+    const int64_t fe = min(nf, fb + chunk);
+    if (fb >= fe) return;
+    const TileDesc d = tiles[tile];
+    const int tid = threadIdx.x;
+    const int E = d.ent;
+
+    uint32_t key[kTileEnt];
+    T v[kTileEnt];
+#pragma unroll
+    for (int e = 0; e < kTileEnt; ++e) {
+        key[e] = 0u;
+        v[e] = T(0);
+        if (e < E) {
+            const int64_t q = d.ent_off + (int64_t)e * kTileThreads + tid;
+            key[e] = keys[q];
+            v[e] = vals[q];
+        }
+    }
+    int64_t xo[kColsPerThread];
+#pragma unroll
+    for (int j = 0; j < kColsPerThread; ++j) {
+        const int u = tid + j * kTileThreads;
+        xo[j] = u < d.ncols ? (int64_t)cols[d.col_off + u] * sc : -1;
+    }
+    const uint16_t *rs = rseg + d.row0 + tile;
+    int sb[kRowsPerThread], se[kRowsPerThread];
+    double rscale[kRowsPerThread];
+#pragma unroll
+    for (int i = 0; i < kRowsPerThread; ++i) {
+        const int r = tid + i * kTileThreads;
+        sb[i] = 0;
+        se[i] = 0;
+        rscale[i] = 1.0;
+        if (r < d.nrows) {
+            sb[i] = rs[r];
+            se[i] = rs[r + 1];
+            if (scale) rscale[i] = scale[d.row0 + r];
+        }
+    }
+
+    T pre[F][kColsPerThread];
+#pragma unroll
+    for (int f = 0; f < F; ++f)
+#pragma unroll
+        for (int j = 0; j < kColsPerThread; ++j)
+            pre[f][j] = (xo[j] >= 0 && fb + f < fe) ? x[xo[j] + (fb + f) * sf] : T(0);
+
+    for (int64_t f0 = fb; f0 < fe; f0 += F) {
+#pragma unroll
+        for (int f = 0; f < F; ++f)
+#pragma unroll
+            for (int j = 0; j < kColsPerThread; ++j) xs[f][tid + j * kTileThreads] = pre[f][j];
+        const int64_t fn = f0 + F;
+        if (fn < fe) {
+#pragma unroll
+            for (int f = 0; f < F; ++f)
+#pragma unroll
+                for (int j = 0; j < kColsPerThread; ++j)
+                    pre[f][j] = (xo[j] >= 0 && fn + f < fe) ? x[xo[j] + (fn + f) * sf] : T(0);
+        }
+        __syncthreads();
+        // segment sums (padding slots carry value 0 and the sink segment id)
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+            SegAcc<T> a;
+            a.init();
+#pragma unroll
+            for (int e = 0; e < kTileEnt; ++e) {
+                if (e < E) {
+                    a.add(v[e], xs[f][key[e] & 0xffffu]);
+                    const uint32_t sid = key[e] >> 16;
+                    const uint32_t nxt = (e + 1 < kTileEnt) ? (key[e + 1] >> 16) : 0xffffffffu;
+                    if (e + 1 == E || nxt != sid) {
+                        seg[f][sid] = a.value();
+                        a.init();
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // rows: add their segments in order, store coalesced
+#pragma unroll
+        for (int i = 0; i < kRowsPerThread; ++i) {
+            const int r = tid + i * kTileThreads;
+            if (r < d.nrows) {
+                const int64_t row = d.row0 + r;
+#pragma unroll
+                for (int f = 0; f < F; ++f) {
+                    if (f0 + f < fe) {
+                        double acc = 0.0;
+                        for (int q = sb[i]; q < se[i]; ++q) acc += (double)seg[f][q];
+                        y[row * dc + (f0 + f) * df] = row_finish(acc, rscale[i], scale != nullptr, (T *)nullptr);
+                    }
+                }
+            }
+        }
+        // the next iteration's LDS writes of xs follow the barrier above (all
+        // segment sums are done); its segment writes follow its first barrier
+        // (all row passes of this iteration are done)
+    }
+}
+
+// ---- plan ----------------------------------------------------------------------------
+const TilePlan &tile_plan(const Csr &A, hipStream_t s) {
+    std::lock_guard<std::mutex> lock(A.tiles_mu);
+    if (A.tiles) return *A.tiles;
+    auto P = std::make_shared<TilePlan>();
+    const int64_t nrows = A.nrows, nnz = A.nnz;
+    std::vector<int64_t> rp(nrows + 1);
+    std::vector<int32_t> col(nnz);
+    std::vector<double> val(nnz);
+    CR_HIP(hipMemcpyAsync(rp.data(), A.rowptr.p, A.rowptr.bytes(), hipMemcpyDeviceToHost, s));
+    if (nnz) {
+        CR_HIP(hipMemcpyAsync(col.data(), A.col.p, A.col.bytes(), hipMemcpyDeviceToHost, s));
+        CR_HIP(hipMemcpyAsync(val.data(), A.val.p, A.val.bytes(), hipMemcpyDeviceToHost, s));
+    }
+    CR_HIP(hipStreamSynchronize(s));
+    HostTilePlan H = make_host_tile_plan(nrows, A.ncols, rp.data(), col.data(), val.data());
+
+    P->ntiles = (int64_t)H.desc.size();
+    P->nslots = (int64_t)H.key.size();
+    P->nhuge = (int64_t)H.huge.size();
+    P->tiles.alloc_kept(H.desc.size(), s);
+    P->key.alloc_kept(H.key.size(), s);
+    P->val32.alloc_kept(H.v32.size(), s);
+    P->val64.alloc_kept(H.v64.size(), s);
+    P->cols.alloc_kept(H.cols.size(), s);
+    P->rseg.alloc_kept(H.rseg.size(), s);
+    P->huge.alloc_kept(H.huge.size(), s);
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+        if (bytes) CR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    };
+    up(P->tiles.p, H.desc.data(), H.desc.size() * sizeof(TileDesc));
+    up(P->key.p, H.key.data(), H.key.size() * sizeof(uint32_t));
+    up(P->val32.p, H.v32.data(), H.v32.size() * sizeof(float));
+    up(P->val64.p, H.v64.data(), H.v64.size() * sizeof(double));
+    up(P->cols.p, H.cols.data(), H.cols.size() * sizeof(int32_t));
+    up(P->rseg.p, H.rseg.data(), H.rseg.size() * sizeof(uint16_t));
+    up(P->huge.p, H.huge.data(), H.huge.size() * sizeof(int32_t));
+    CR_HIP(hipStreamSynchronize(s));
+    A.tiles = P;
+    return *A.tiles;
+}
+
+void drop_tile_plan(const Csr &A) {
+    std::lock_guard<std::mutex> lock(A.tiles_mu);
+    A.tiles.reset();
+}
+
+// ---- launch --------------------------------------------------------------------------
+template <class T, int F>
+static void launch_f(const TilePlan &P, int nchunks, const T *x, int64_t sc, int64_t sf, T *y,
+                     int64_t dc, int64_t df, int64_t nf, const double *scale, hipStream_t s) {
+    const int64_t tiles_per_xcd = (P.ntiles + 7) / 8;
+    int64_t chunk = (nf + nchunks - 1) / nchunks;
+    chunk = (chunk + F - 1) / F * F;
+    nchunks = (int)((nf + chunk - 1) / chunk);
+    const int64_t blocks = tiles_per_xcd * 8 * nchunks;
+    CR_REQUIRE(blocks < (int64_t)1 << 31, CR_UNSUPPORTED, "too many tiles x field chunks for one launch");
+    const T *vals;
+    if constexpr (sizeof(T) == 4) vals = (const T *)P.val32.p;
+    else vals = (const T *)P.val64.p;
+    hipLaunchKernelGGL((k_spmm_tiles<T, F>), dim3((unsigned)blocks), dim3(kTileThreads), 0, s, P.tiles.p,
+                       P.ntiles, tiles_per_xcd, nchunks, chunk, P.key.p, vals, P.cols.p, P.rseg.p, x, sc, sf,
+                       y, dc, df, nf, scale);
+}
+
+// Workgroups for a tile launch: enough (tile, chunk) pairs to keep every CU
+// busy several times over when the plan has few tiles.
+int default_chunks(const TilePlan &P, int64_t nf, int F) {
+    const int64_t want = 8 * 256;  // 8 workgroups per CU
+    int64_t c = (want + P.ntiles - 1) / std::max<int64_t>(1, P.ntiles);
+    c = std::max<int64_t>(1, std::min<int64_t>(c, (nf + F - 1) / F));
+    return (int)c;
+}
+
+template <class T>
+void launch_tiles(const TilePlan &P, int F, int nchunks, const T *x, int64_t sc, int64_t sf, T *y,
+                  int64_t dc, int64_t df, int64_t nf, const double *scale, hipStream_t s) {
+    if (P.ntiles == 0) return;
+    if (nchunks <= 0) nchunks = default_chunks(P, nf, F);
+    switch (F) {
+        case 1: launch_f<T, 1>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, s); break;
+        case 2: launch_f<T, 2>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, s); break;
+        case 4: launch_f<T, 4>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, s); break;
+        default:
+            if constexpr (sizeof(T) == 4) launch_f<T, 8>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, s);
+            else launch_f<T, 4>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, s);
+            break;
+    }
+}
+
+template void launch_tiles<float>(const TilePlan &, int, int, const float *, int64_t, int64_t, float *,
+                                  int64_t, int64_t, int64_t, const double *, hipStream_t);
+template void launch_tiles<double>(const TilePlan &, int, int, const double *, int64_t, int64_t, double *,
+                                   int64_t, int64_t, int64_t, const double *, hipStream_t);
+
+}  // namespace cr

@@ -422,6 +422,20 @@ def regrid_tune(dst_field, R: Regridder, src_field, *, dims: int = 1):
     return chosen.value, ms.value
 
 
+def regrid_variants(dtype="float32") -> int:
+    """Number of apply kernel variants for FP32 (``"float32"``) or FP64 fields."""
+    return int(L.lib.cr_regrid_variant_count(_dtype_code(dtype)))
+
+
+def regrid_set_variant(R: Regridder, variant: int, dtype="float32") -> None:
+    """Pin the apply kernel variant used for this direction of ``R`` and this
+    field dtype (``-1`` restores the default).  FP32: 0-4 row kernel with 1-16
+    fields per thread, 5-7 staged tile kernel with 2/4/8 fields per stage;
+    FP64: 0 row kernel, 1 tile kernel."""
+    L.check(L.lib.cr_regrid_set_variant(R._core.h, 1 if R._t else 0, _dtype_code(dtype), int(variant)),
+            "regrid_set_variant")
+
+
 def regrid_backward(R: Regridder, dst_grid, src_grid, x, y, gy, *, normalize: bool = True,
                     field_grad: bool = True, vertex_grad: bool = True):
     """Reverse mode of ``y = regrid(R, x)`` (FP64 device tensors, cells along
