@@ -173,6 +173,26 @@ def main():
     log(f"[bench] batched FP32 apply: {ms_step:.4f} ms/step, {gbps:.0f} GB/s algorithmic "
         f"({B32 / 1e9:.3f} GB/launch), wall {wall_steps:.3f} s")
 
+    # the same launch captured in a HIP graph (torch.cuda.CUDAGraph), replayed
+    graph = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        launch()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(graph):
+        launch()
+    graph.replay()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(args.steps):
+        graph.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_graph = e0.elapsed_time(e1) / args.steps
+    log(f"[bench] batched FP32 apply, graph replay: {ms_graph:.4f} ms/step")
+
     # ---- loop of 256 plain FP64 single-field applies --------------------------
     # Timed for each FP64 kernel variant (row kernel, tile kernel); the fastest
     # is the plain-FP64 figure the batched launch is compared with.
@@ -286,12 +306,14 @@ def main():
         "build": {"pairs_per_s": round(nnz / build_s, 1), "ms": round(build_s * 1e3, 3), "nnz": nnz,
                   "candidates": ncand, "candidate_pairs_per_s": round(ncand / build_s, 1),
                   "stage_ms": {k: round(st[k], 3) for k in
-                               ("ms_bounds", "ms_tree", "ms_search", "ms_clip", "ms_assemble", "ms_total")},
+                               ("ms_bounds", "ms_tree", "ms_search", "ms_clip", "ms_assemble", "ms_total",
+                                "ms_host")},
                   "speedup_vs_cpu": None if build_speedup is None else round(build_speedup, 2)},
         "fp64_loop": {"ms": round(ms_loop, 4), "GBps": round(gbps_loop, 2),
                       "batched_speedup": round(ms_loop / ms_step, 3), "kernel": best64,
                       "ms_by_kernel": {k: round(v, 4) for k, v in loop_ms.items()}},
         "copy_GBps": round(copy_gbps, 1),
+        "graph_ms_per_step": round(ms_graph, 5),
         "accuracy": {"fp32_max_rel_field_error": rel_err, "fp32_conservation_drift": drift,
                      "fp64_conservation_drift": drift64},
         "roofline": {"bound": "hbm", "achieved": round(gbps, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",

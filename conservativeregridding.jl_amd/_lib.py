@@ -41,7 +41,7 @@ class BuildStats(ctypes.Structure):
     _fields_ = [("n_dst", c_i64), ("n_src", c_i64), ("nnz", c_i64), ("n_candidates", c_i64),
                 ("ms_total", c_dbl), ("ms_bounds", c_dbl), ("ms_tree", c_dbl), ("ms_search", c_dbl),
                 ("ms_clip", c_dbl), ("ms_assemble", c_dbl), ("clip_with_dst", c_i32),
-                ("flipped_cells", c_i32)]
+                ("flipped_cells", c_i32), ("ms_host", c_dbl)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

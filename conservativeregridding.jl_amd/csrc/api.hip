@@ -1,5 +1,6 @@
 // api.hip -- library-level entry points: errors, Regridder construction from
 // CSR arrays, queries and host copies.
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -126,14 +127,40 @@ cr_status cr_regridder_stats(const cr_regridder *R, cr_build_stats *st) {
     });
 }
 
+// Host copy with the columns of every row ascending (the order sparse() gives
+// the reference's matrix); the device keeps each row in candidate order.
 cr_status cr_regridder_copy_csr(const cr_regridder *R, int32_t transpose, int64_t *rowptr,
                                 int32_t *col, double *val) {
     return guard([&] {
         CR_REQUIRE(R, CR_INVALID_ARGUMENT, "null regridder");
         const Csr &A = transpose ? R->bwd : R->fwd;
-        if (rowptr) CR_HIP(hipMemcpy(rowptr, A.rowptr.p, A.rowptr.bytes(), hipMemcpyDeviceToHost));
-        if (col && A.nnz) CR_HIP(hipMemcpy(col, A.col.p, A.col.bytes(), hipMemcpyDeviceToHost));
-        if (val && A.nnz) CR_HIP(hipMemcpy(val, A.val.p, A.val.bytes(), hipMemcpyDeviceToHost));
+        std::vector<int64_t> rp(A.nrows + 1);
+        std::vector<int32_t> c(A.nnz);
+        std::vector<double> v(A.nnz);
+        CR_HIP(hipMemcpy(rp.data(), A.rowptr.p, A.rowptr.bytes(), hipMemcpyDeviceToHost));
+        if (A.nnz) {
+            CR_HIP(hipMemcpy(c.data(), A.col.p, A.col.bytes(), hipMemcpyDeviceToHost));
+            CR_HIP(hipMemcpy(v.data(), A.val.p, A.val.bytes(), hipMemcpyDeviceToHost));
+        }
+        std::vector<int32_t> order;
+        for (int64_t i = 0; i < A.nrows; ++i) {
+            const int64_t k0 = rp[i], n = rp[i + 1] - rp[i];
+            bool sorted = true;
+            for (int64_t k = 1; k < n && sorted; ++k) sorted = c[k0 + k - 1] < c[k0 + k];
+            if (sorted) {
+                if (col) std::copy(c.begin() + k0, c.begin() + k0 + n, col + k0);
+                if (val) std::copy(v.begin() + k0, v.begin() + k0 + n, val + k0);
+                continue;
+            }
+            order.resize(n);
+            for (int64_t k = 0; k < n; ++k) order[k] = (int32_t)k;
+            std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return c[k0 + a] < c[k0 + b]; });
+            for (int64_t k = 0; k < n; ++k) {
+                if (col) col[k0 + k] = c[k0 + order[k]];
+                if (val) val[k0 + k] = v[k0 + order[k]];
+            }
+        }
+        if (rowptr) std::copy(rp.begin(), rp.end(), rowptr);
     });
 }
 

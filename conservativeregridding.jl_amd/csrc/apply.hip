@@ -195,34 +195,34 @@ __global__ void k_inv(const double *__restrict__ a, int64_t n, double *__restric
 // Rows with more than kLongRow non-zeros, appended in any order (each long row
 // is reduced by its own wave, so the list order does not affect results).
 __global__ void k_long_rows(const int64_t *__restrict__ rowptr, int64_t n, int32_t *__restrict__ list,
-                            unsigned long long *__restrict__ stat) {
+                            unsigned long long *__restrict__ count) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    unsigned long long nnz = (unsigned long long)(rowptr[i + 1] - rowptr[i]);
-    if (nnz > (unsigned long long)kLongRow) list[atomicAdd(&stat[0], 1ull)] = (int32_t)i;
-    atomicMax(&stat[1], nnz);
+    if (rowptr[i + 1] - rowptr[i] > kLongRow) list[atomicAdd(count, 1ull)] = (int32_t)i;
 }
 
-static void plan_long_rows(Csr &A, hipStream_t s) {
-    A.long_rows.release();
-    A.nlong = 0;
-    A.max_row_nnz = 0;
-    if (A.nrows == 0) return;
-    DevBuf<int32_t> list(A.nrows, s);
-    DevBuf<unsigned long long> stat(2, s);
-    CR_HIP(hipMemsetAsync(stat.p, 0, stat.bytes(), s));
-    hipLaunchKernelGGL(k_long_rows, dim3(grid_for(A.nrows, 256)), dim3(256), 0, s, A.rowptr.p, A.nrows,
-                       list.p, stat.p);
-    CR_HIP(hipGetLastError());
-    unsigned long long h[2] = {0, 0};
-    CR_HIP(hipMemcpyAsync(h, stat.p, sizeof(h), hipMemcpyDeviceToHost, s));
-    CR_HIP(hipStreamSynchronize(s));
-    A.nlong = (int64_t)h[0];
-    A.max_row_nnz = (int64_t)h[1];
-    if (A.nlong) {
-        A.long_rows.alloc_kept(A.nlong, s);
-        CR_HIP(hipMemcpyAsync(A.long_rows.p, list.p, A.nlong * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+// The long-row list of A, made on the first row-kernel launch (synchronises s once).
+static void ensure_long_rows(const Csr &A, hipStream_t s) {
+    std::lock_guard<std::mutex> lock(A.plan_mu);
+    if (A.nlong >= 0) return;
+    int64_t nlong = 0;
+    if (A.nrows) {
+        DevBuf<int32_t> list(A.nrows, s);
+        DevBuf<unsigned long long> count(1, s);
+        CR_HIP(hipMemsetAsync(count.p, 0, count.bytes(), s));
+        hipLaunchKernelGGL(k_long_rows, dim3(grid_for(A.nrows, 256)), dim3(256), 0, s, A.rowptr.p, A.nrows,
+                           list.p, count.p);
+        CR_HIP(hipGetLastError());
+        unsigned long long h = 0;
+        CR_HIP(hipMemcpyAsync(&h, count.p, sizeof(h), hipMemcpyDeviceToHost, s));
+        CR_HIP(hipStreamSynchronize(s));
+        nlong = (int64_t)h;
+        if (nlong) {
+            A.long_rows.alloc_kept(nlong, s);
+            CR_HIP(hipMemcpyAsync(A.long_rows.p, list.p, nlong * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+        }
     }
+    A.nlong = nlong;
 }
 
 void finalize_regridder(cr_regridder *R, hipStream_t s) {
@@ -233,8 +233,6 @@ void finalize_regridder(cr_regridder *R, hipStream_t s) {
     hipLaunchKernelGGL(k_inv, dim3(grid_for(R->bwd.nrows, 256)), dim3(256), 0, s, R->a_src.p,
                        R->bwd.nrows, R->bwd.inv_area.p);
     CR_HIP(hipGetLastError());
-    plan_long_rows(R->fwd, s);
-    plan_long_rows(R->bwd, s);
 }
 
 // ---- launch ------------------------------------------------------------------------
@@ -245,6 +243,7 @@ static void launch_rows(const Csr &A, const T *x, int64_t sc, int64_t sf, T *y, 
     int64_t tiles_per_xcd = (ntiles + 7) / 8;
     int64_t gy = (nf + FB - 1) / FB;
     CR_REQUIRE(gy <= 65535, CR_UNSUPPORTED, "too many fields for one launch");
+    ensure_long_rows(A, s);
     hipLaunchKernelGGL((k_apply_rows<T, FB>), dim3((unsigned)(tiles_per_xcd * 8), (unsigned)gy),
                        dim3(kApplyBlock), 0, s, A.rowptr.p, A.col.p, A.val.p, A.nrows, x, sc, sf, y, dc,
                        df, nf, area, inv, tiles_per_xcd);

@@ -13,14 +13,16 @@
 //     -> k_clip<4>: one thread per candidate, streaming FP64 clip + area
 //   keep iff area > 0 (here > threshold_rel * min(a_src, a_dst)) and
 //   sparse(dst, src, areas)
-//     -> k_keep_scatter: wave64 ballot compaction of kept pairs (row-grouped COO)
-//        two stable radix sorts: by column (gives CSR(A^T)) then by row (gives
-//        CSR(A) with sorted columns)
+//     -> k_keep_count / k_keep_scatter: wave64 ballot compaction of kept pairs;
+//        the rows of CSR(A) stay grouped as the candidates were (row pointers
+//        from the keep mask), one stable radix sort by column gives CSR(A^T)
+//        (host copies of CSR(A) sort each row's columns, like sparse())
 //   areas(manifold, tree)  -> the k_cell_bounds areas, times radius^2
 //   normalize!             -> k_scale by 1 / max(A)
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <vector>
 
 #include "regridder.hpp"
@@ -291,13 +293,17 @@ constexpr int kCompactBlock = 256;
 
 __global__ void __launch_bounds__(kCompactBlock) k_keep_count(const double *__restrict__ val,
                                                               int64_t n,
-                                                              int64_t *__restrict__ block_cnt) {
+                                                              int64_t *__restrict__ block_cnt,
+                                                              uint64_t *__restrict__ keep_mask) {
     __shared__ int wave_cnt[kCompactBlock / 64];
     int64_t c = (int64_t)blockIdx.x * kCompactBlock + threadIdx.x;
     bool keep = c < n && val[c] > 0.0;
     uint64_t m = __ballot(keep);
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) wave_cnt[wave] = __popcll(m);
+    if (lane == 0) {
+        wave_cnt[wave] = __popcll(m);
+        keep_mask[(int64_t)blockIdx.x * (kCompactBlock / 64) + wave] = m;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         int t = 0;
@@ -309,8 +315,7 @@ __global__ void __launch_bounds__(kCompactBlock) k_keep_count(const double *__re
 __global__ void __launch_bounds__(kCompactBlock) k_keep_scatter(
     const double *__restrict__ val, const int32_t *__restrict__ cand_dst,
     const int32_t *__restrict__ cand_src, int64_t n, const int64_t *__restrict__ block_off,
-    int32_t *__restrict__ out_row, int32_t *__restrict__ out_col, double *__restrict__ out_val,
-    unsigned long long *__restrict__ row_cnt) {
+    int32_t *__restrict__ out_row, int32_t *__restrict__ out_col, double *__restrict__ out_val) {
     __shared__ int wave_base[kCompactBlock / 64];
     int64_t c = (int64_t)blockIdx.x * kCompactBlock + threadIdx.x;
     double v = c < n ? val[c] : 0.0;
@@ -335,14 +340,45 @@ __global__ void __launch_bounds__(kCompactBlock) k_keep_scatter(
         out_row[o] = r;
         out_col[o] = cand_src[c];
         out_val[o] = v;
-        atomicAdd(&row_cnt[r], 1ull);
     }
 }
 
-__global__ void k_histogram(const int32_t *__restrict__ key, int64_t n,
-                            unsigned long long *__restrict__ cnt) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicAdd(&cnt[key[i]], 1ull);
+// Row pointers of the compacted pairs: candidates are grouped by destination
+// row (row d owns candidates [cand_off[d], cand_off[d+1])), so row d starts at
+// the number of kept candidates before cand_off[d] = block offset + popcounts
+// of the keep mask inside the block.
+__global__ void k_rowptr_from_mask(const int64_t *__restrict__ cand_off, int64_t nrows, int64_t ncand,
+                                   const int64_t *__restrict__ block_off,
+                                   const uint64_t *__restrict__ keep_mask, int64_t *__restrict__ rowptr) {
+    int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d > nrows) return;
+    const int64_t k = cand_off[d];
+    if (k >= ncand) {
+        rowptr[d] = block_off[(ncand + kCompactBlock - 1) / kCompactBlock];
+        return;
+    }
+    const int64_t b = k / kCompactBlock;
+    int64_t p = block_off[b];
+    const int64_t w0 = b * (kCompactBlock / 64), kw = k >> 6;
+    for (int64_t w = w0; w < kw; ++w) p += __popcll(keep_mask[w]);
+    const int bit = (int)(k & 63);
+    if (bit) p += __popcll(keep_mask[kw] & ((1ull << bit) - 1ull));
+    rowptr[d] = p;
+}
+
+// Row pointers of a matrix whose entries are sorted by row key: rowptr[r] =
+// first position with key >= r (binary search; empty rows need no care).
+__global__ void k_rowptr_from_sorted(const int32_t *__restrict__ key, int64_t n, int64_t nrows,
+                                     int64_t *__restrict__ rowptr) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > nrows) return;
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (key[mid] < r) lo = mid + 1;
+        else hi = mid;
+    }
+    rowptr[r] = lo;
 }
 
 __global__ void k_iota(int32_t *__restrict__ x, int64_t n) {
@@ -513,15 +549,16 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     }
     tm.mark();  // 4
 
-    // 5. compaction (row-grouped COO) and the two CSRs
+    // 5. compaction (rows stay grouped in candidate order) and the two CSRs
     int64_t nblk = (ncand + kCompactBlock - 1) / kCompactBlock;
     DevBuf<int64_t> blk(nblk + 1, s);
+    DevBuf<uint64_t> keep_mask(nblk * (kCompactBlock / 64) + 1, s);
     {
         DevBuf<int64_t> blk_cnt(nblk + 1, s);
         CR_HIP(hipMemsetAsync(blk_cnt.p, 0, blk_cnt.bytes(), s));
         if (ncand)
             hipLaunchKernelGGL(k_keep_count, dim3((unsigned)nblk), dim3(kCompactBlock), 0, s, cval.p,
-                               ncand, blk_cnt.p);
+                               ncand, blk_cnt.p, keep_mask.p);
         exclusive_sum(T, blk_cnt.p, blk.p, nblk + 1, s);
     }
     int64_t nnz = 0;
@@ -539,35 +576,28 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     Bk.col.alloc_kept(nnz, s);
     Bk.val.alloc_kept(nnz, s);
     {
-        DevBuf<int32_t> coo_row(nnz, s), coo_col(nnz, s);
-        DevBuf<double> coo_val(nnz, s);
-        DevBuf<unsigned long long> rcnt(nd + 1, s), ccnt(ns + 1, s);
-        CR_HIP(hipMemsetAsync(rcnt.p, 0, rcnt.bytes(), s));
-        CR_HIP(hipMemsetAsync(ccnt.p, 0, ccnt.bytes(), s));
+        // CSR(A): kept pairs in candidate order (columns in search order within a row)
+        DevBuf<int32_t> coo_row(nnz, s);
         if (ncand)
             hipLaunchKernelGGL(k_keep_scatter, dim3((unsigned)nblk), dim3(kCompactBlock), 0, s, cval.p,
-                               cand_dst.p, cand_src.p, ncand, blk.p, coo_row.p, coo_col.p, coo_val.p,
-                               rcnt.p);
-        hipLaunchKernelGGL(k_histogram, dim3(grid_for(nnz, B)), dim3(B), 0, s, coo_col.p, nnz, ccnt.p);
+                               cand_dst.p, cand_src.p, ncand, blk.p, coo_row.p, F.col.p, F.val.p);
+        hipLaunchKernelGGL(k_rowptr_from_mask, dim3(grid_for(nd + 1, B)), dim3(B), 0, s, cand_off.p, nd, ncand,
+                           blk.p, keep_mask.p, F.rowptr.p);
         CR_HIP(hipGetLastError());
-        exclusive_sum(T, (int64_t *)rcnt.p, F.rowptr.p, nd + 1, s);
-        exclusive_sum(T, (int64_t *)ccnt.p, Bk.rowptr.p, ns + 1, s);
+        // CSR(A^T): stable sort by column keeps the rows of A ascending inside each row of A^T
         if (nnz) {
-            // stable sort by column: CSR(A^T), rows of A^T sorted by A's row
-            DevBuf<int32_t> iota(nnz, s), p1(nnz, s), tkey(nnz, s), p2(nnz, s), fkey(nnz, s);
+            DevBuf<int32_t> iota(nnz, s), perm_t(nnz, s), tkey(nnz, s);
             hipLaunchKernelGGL(k_iota, dim3(grid_for(nnz, B)), dim3(B), 0, s, iota.p, nnz);
-            sort_pairs(T, coo_col.p, tkey.p, iota.p, p1.p, nnz, bits_for(ns), s);
-            hipLaunchKernelGGL(k_gather<int32_t>, dim3(grid_for(nnz, B)), dim3(B), 0, s, coo_row.p, p1.p,
+            sort_pairs(T, F.col.p, tkey.p, iota.p, perm_t.p, nnz, bits_for(ns), s);
+            hipLaunchKernelGGL(k_gather<int32_t>, dim3(grid_for(nnz, B)), dim3(B), 0, s, coo_row.p, perm_t.p,
                                nnz, Bk.col.p);
-            hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(nnz, B)), dim3(B), 0, s, coo_val.p, p1.p,
+            hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(nnz, B)), dim3(B), 0, s, F.val.p, perm_t.p,
                                nnz, Bk.val.p);
-            // stable sort of that by row: CSR(A) with sorted columns
-            sort_pairs(T, Bk.col.p, fkey.p, iota.p, p2.p, nnz, bits_for(nd), s);
-            hipLaunchKernelGGL(k_gather<int32_t>, dim3(grid_for(nnz, B)), dim3(B), 0, s, tkey.p, p2.p,
-                               nnz, F.col.p);
-            hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(nnz, B)), dim3(B), 0, s, Bk.val.p, p2.p,
-                               nnz, F.val.p);
+            hipLaunchKernelGGL(k_rowptr_from_sorted, dim3(grid_for(ns + 1, B)), dim3(B), 0, s, tkey.p, nnz, ns,
+                               Bk.rowptr.p);
             CR_HIP(hipGetLastError());
+        } else {
+            CR_HIP(hipMemsetAsync(Bk.rowptr.p, 0, Bk.rowptr.bytes(), s));
         }
     }
 
@@ -656,11 +686,14 @@ cr_status cr_regridder_build(cr_regridder **R, const cr_grid *dst, const cr_grid
         CR_REQUIRE(o.radius > 0.0, CR_INVALID_ARGUMENT, "radius must be > 0");
         CR_REQUIRE(o.clip_side >= 0 && o.clip_side <= 2, CR_INVALID_ARGUMENT, "bad clip_side");
         check_device();
+        const auto t0 = std::chrono::steady_clock::now();
         hipStream_t s = as_stream(stream);
         auto *r = new cr_regridder();
         try {
             build_regridder(r, dst, src, o, s);
             if (o.normalize) normalize_regridder(r, s);
+            r->stats.ms_host =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         } catch (...) {
             delete r;
             throw;

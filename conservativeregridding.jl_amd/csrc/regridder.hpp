@@ -37,14 +37,14 @@ struct Csr {
     DevBuf<int32_t> col;
     DevBuf<double> val;
     DevBuf<double> inv_area;   // 1 / area of the row cells (FP32 batched path)
-    DevBuf<int32_t> long_rows; // rows with more than kLongRow non-zeros (own kernel)
-    int64_t nlong = 0;
-    int64_t max_row_nnz = 0;
     int tuned_variant = -1;    // FP32 kernel variant (cr_regrid_tune / set_variant); -1: default
     int f64_variant = -1;      // FP64 kernel variant; -1: default
-    // staged-apply plan, made on first use (guarded: handles are shared by callers)
+    // Kernel plans, made on first use (guarded: handles are shared by callers):
+    // the staged-apply tiles, and the rows too long for the row kernels.
     mutable std::shared_ptr<TilePlan> tiles;
-    mutable std::mutex tiles_mu;
+    mutable DevBuf<int32_t> long_rows;
+    mutable int64_t nlong = -1;  // -1: not planned yet
+    mutable std::mutex plan_mu;
 };
 
 // The tile plan of A, built on the first call (synchronises stream s once).

@@ -71,6 +71,13 @@ __device__ __forceinline__ double row_finish(double acc, double scale, bool has,
     return has ? acc / scale : acc;
 }
 
+// F field values of one cell (or one segment) side by side, so one LDS
+// instruction moves all F of them.
+template <class T, int F>
+struct alignas(sizeof(T) * F >= 16 ? 16 : sizeof(T) * F) Pack {
+    T v[F];
+};
+
 template <class T, int F>
 __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
     const TileDesc *__restrict__ tiles, int64_t ntiles, int64_t tiles_per_xcd, int nchunks,
@@ -78,8 +85,8 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
     const int32_t *__restrict__ cols, const uint16_t *__restrict__ rseg, const T *__restrict__ x,
     int64_t sc, int64_t sf, T *__restrict__ y, int64_t dc, int64_t df, int64_t nf,
     const double *__restrict__ scale) {
-    __shared__ T xs[F][kTileCols];
-    __shared__ T seg[F][kTileSegs + 1];
+    __shared__ Pack<T, F> xs[kTileCols];
+    __shared__ Pack<T, F> seg[kTileSegs + 1];
     const int64_t b = blockIdx.x;
     const int64_t local = b >> 3;
     const int64_t tile = (b & 7) * tiles_per_xcd + local / nchunks;
@@ -102,6 +109,13 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
             key[e] = keys[q];
             v[e] = vals[q];
         }
+    }
+    // flush[e]: entry e ends its segment (the next slot starts another one)
+    uint32_t flush = 0u;
+#pragma unroll
+    for (int e = 0; e < kTileEnt; ++e) {
+        const uint32_t nxt = (e + 1 < kTileEnt) ? (key[e + 1] >> 16) : 0xffffffffu;
+        if (e < E && (e + 1 == E || nxt != (key[e] >> 16))) flush |= 1u << e;
     }
     int64_t xo[kColsPerThread];
 #pragma unroll
@@ -134,9 +148,12 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
 
     for (int64_t f0 = fb; f0 < fe; f0 += F) {
 #pragma unroll
-        for (int f = 0; f < F; ++f)
+        for (int j = 0; j < kColsPerThread; ++j) {
+            Pack<T, F> p;
 #pragma unroll
-            for (int j = 0; j < kColsPerThread; ++j) xs[f][tid + j * kTileThreads] = pre[f][j];
+            for (int f = 0; f < F; ++f) p.v[f] = pre[f][j];
+            xs[tid + j * kTileThreads] = p;
+        }
         const int64_t fn = f0 + F;
         if (fn < fe) {
 #pragma unroll
@@ -146,20 +163,25 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
                     pre[f][j] = (xo[j] >= 0 && fn + f < fe) ? x[xo[j] + (fn + f) * sf] : T(0);
         }
         __syncthreads();
-        // segment sums (padding slots carry value 0 and the sink segment id)
+        // segment sums for F fields at once (padding slots: value 0, sink segment)
+        {
+            SegAcc<T> a[F];
 #pragma unroll
-        for (int f = 0; f < F; ++f) {
-            SegAcc<T> a;
-            a.init();
+            for (int f = 0; f < F; ++f) a[f].init();
 #pragma unroll
             for (int e = 0; e < kTileEnt; ++e) {
                 if (e < E) {
-                    a.add(v[e], xs[f][key[e] & 0xffffu]);
-                    const uint32_t sid = key[e] >> 16;
-                    const uint32_t nxt = (e + 1 < kTileEnt) ? (key[e + 1] >> 16) : 0xffffffffu;
-                    if (e + 1 == E || nxt != sid) {
-                        seg[f][sid] = a.value();
-                        a.init();
+                    const Pack<T, F> xv = xs[key[e] & 0xffffu];
+#pragma unroll
+                    for (int f = 0; f < F; ++f) a[f].add(v[e], xv.v[f]);
+                    if (flush & (1u << e)) {
+                        Pack<T, F> o;
+#pragma unroll
+                        for (int f = 0; f < F; ++f) {
+                            o.v[f] = a[f].value();
+                            a[f].init();
+                        }
+                        seg[key[e] >> 16] = o;
                     }
                 }
             }
@@ -171,14 +193,18 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
             const int r = tid + i * kTileThreads;
             if (r < d.nrows) {
                 const int64_t row = d.row0 + r;
+                double acc[F];
 #pragma unroll
-                for (int f = 0; f < F; ++f) {
-                    if (f0 + f < fe) {
-                        double acc = 0.0;
-                        for (int q = sb[i]; q < se[i]; ++q) acc += (double)seg[f][q];
-                        y[row * dc + (f0 + f) * df] = row_finish(acc, rscale[i], scale != nullptr, (T *)nullptr);
-                    }
+                for (int f = 0; f < F; ++f) acc[f] = 0.0;
+                for (int q = sb[i]; q < se[i]; ++q) {
+                    const Pack<T, F> sp = seg[q];
+#pragma unroll
+                    for (int f = 0; f < F; ++f) acc[f] += (double)sp.v[f];
                 }
+#pragma unroll
+                for (int f = 0; f < F; ++f)
+                    if (f0 + f < fe)
+                        y[row * dc + (f0 + f) * df] = row_finish(acc[f], rscale[i], scale != nullptr, (T *)nullptr);
             }
         }
         // the next iteration's LDS writes of xs follow the barrier above (all
@@ -189,7 +215,7 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
 
 // ---- plan ----------------------------------------------------------------------------
 const TilePlan &tile_plan(const Csr &A, hipStream_t s) {
-    std::lock_guard<std::mutex> lock(A.tiles_mu);
+    std::lock_guard<std::mutex> lock(A.plan_mu);
     if (A.tiles) return *A.tiles;
     auto P = std::make_shared<TilePlan>();
     const int64_t nrows = A.nrows, nnz = A.nnz;
@@ -230,7 +256,7 @@ const TilePlan &tile_plan(const Csr &A, hipStream_t s) {
 }
 
 void drop_tile_plan(const Csr &A) {
-    std::lock_guard<std::mutex> lock(A.tiles_mu);
+    std::lock_guard<std::mutex> lock(A.plan_mu);
     A.tiles.reset();
 }
 

@@ -74,6 +74,7 @@ typedef struct {
     double ms_total, ms_bounds, ms_tree, ms_search, ms_clip, ms_assemble;
     int32_t clip_with_dst; /* 1: dst cells were the clip polygons */
     int32_t flipped_cells; /* cells re-oriented to CCW at grid creation */
+    double ms_host;        /* host wall time of the whole build call */
 } cr_build_stats;
 
 /* ------------------------------------------------------------ library -- */
