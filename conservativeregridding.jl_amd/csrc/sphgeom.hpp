@@ -165,7 +165,15 @@ CR_HD double clip_area_stream(const double *clip, int nc, const double *subj, in
     for (int k = 0; k < NC; ++k) c[k] = load3(clip + 3 * (k < nc ? k : nc - 1));
 #pragma unroll
     for (int k = 0; k < NC; ++k) P.set_edge(k, c[k], c[(k + 1) % NC]);
-    for (int i = 0; i < ns; ++i) pipe_push<0, NC>(P, load3(subj + 3 * i));
+    // subject vertices are loaded two ahead of the one being clipped, so their
+    // latency overlaps the pipeline's work
+    d3 v0 = load3(subj), v1 = load3(subj + 3 * (ns > 1 ? 1 : 0));
+    for (int i = 0; i < ns; ++i) {
+        const d3 v2 = load3(subj + 3 * (i + 2 < ns ? i + 2 : ns - 1));
+        pipe_push<0, NC>(P, v0);
+        v0 = v1;
+        v1 = v2;
+    }
     pipe_close<0, NC>(P);
     return P.acc.area();
 }
