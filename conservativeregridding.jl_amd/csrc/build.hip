@@ -4,8 +4,9 @@
 // src/regridder/intersection_areas.jl:67-157) and what replaces each step:
 //   treeify + dual-tree DFS over spherical caps (MultithreadedDualDepthFirstSearch.jl)
 //     -> k_cell_bounds: FP64 area + bounding cap of every cell
-//        Morton sort of source caps + implicit 8-ary tree of caps (k_build_level)
-//        k_search<false/true>: one wave per Morton cluster of 64 destination
+//        space-filling-curve sort (cube face + Hilbert) of source caps +
+//        implicit 8-ary tree of caps (k_build_level)
+//        k_search<false/true>: one wave per curve-order cluster of 64 destination
 //        cells walks the tree with the cluster's cap, then every lane tests its
 //        cell against the cluster's leaf list; count pass + scan + fill pass
 //        emit candidates grouped by destination row in a deterministic order
@@ -42,11 +43,11 @@ __global__ void k_cell_bounds(const double *__restrict__ cells, int64_t n, int n
     cap[c] = cell_cap(p, nv);
 }
 
-__global__ void k_morton(const float4 *__restrict__ cap, int64_t n, uint32_t *__restrict__ key,
+__global__ void k_sfc_keys(const float4 *__restrict__ cap, int64_t n, uint32_t *__restrict__ key,
                          int32_t *__restrict__ idx) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
-    key[c] = morton_key(cap[c]);
+    key[c] = sfc_key(cap[c]);
     idx[c] = (int32_t)c;
 }
 
@@ -64,8 +65,8 @@ __global__ void k_build_level(const float4 *__restrict__ child, int64_t nchild,
     parent[p] = parent_cap(child, c0, c1);
 }
 
-// Candidate search, one wave per 64 destination cells taken in Morton order of
-// their caps (a compact cluster).  The wave bounds its 64 caps by one cap,
+// Candidate search, one wave per 64 destination cells taken in space-filling-
+// curve order of their caps (a compact cluster).  The wave bounds its 64 caps by one cap,
 // walks the source tree level by level with that cap (lanes test the children
 // of the current frontier side by side; ballot + prefix count compact the
 // survivors into LDS), and ends with the list of source leaves that touch the
@@ -77,7 +78,7 @@ __global__ void k_build_level(const float4 *__restrict__ child, int64_t nchild,
 // leaves whose caps touch the cell's cap, in a fixed order.  FILL=false
 // counts; FILL=true writes at the scanned offsets (never past the cell's
 // segment).
-constexpr int kSearchLeaves = 512;
+constexpr int kSearchLeaves = 1024;
 constexpr int kSearchFront = 256;
 
 __device__ __forceinline__ float wave_sum(float v) {
@@ -492,8 +493,8 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     {
         DevBuf<uint32_t> key(ns, s), key_sorted(ns, s);
         DevBuf<int32_t> idx(ns, s);
-        hipLaunchKernelGGL(k_morton, dim3(grid_for(ns, B)), dim3(B), 0, s, cap_src.p, ns, key.p, idx.p);
-        sort_pairs(T, key.p, key_sorted.p, idx.p, perm.p, ns, 30, s);
+        hipLaunchKernelGGL(k_sfc_keys, dim3(grid_for(ns, B)), dim3(B), 0, s, cap_src.p, ns, key.p, idx.p);
+        sort_pairs(T, key.p, key_sorted.p, idx.p, perm.p, ns, 2 * kSfcOrder + 3, s);
         hipLaunchKernelGGL(k_gather_caps, dim3(grid_for(ns, B)), dim3(B), 0, s, cap_src.p, perm.p, ns,
                            nodes.p);
         for (int L = 1; L <= lv.top; ++L)
@@ -503,15 +504,15 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     }
     tm.mark();  // 2
 
-    // 3. candidate search: destination cells in Morton order, count, scan, fill
+    // 3. candidate search: destination cells in curve order, count, scan, fill
     DevBuf<int64_t> cand_off(nd + 1, s);
     DevBuf<int32_t> qorder(nd, s);
     const unsigned qblocks = grid_for(nd, 64);
     {
         DevBuf<uint32_t> key(nd, s), key_sorted(nd, s);
         DevBuf<int32_t> idx(nd, s);
-        hipLaunchKernelGGL(k_morton, dim3(grid_for(nd, B)), dim3(B), 0, s, cap_dst.p, nd, key.p, idx.p);
-        sort_pairs(T, key.p, key_sorted.p, idx.p, qorder.p, nd, 30, s);
+        hipLaunchKernelGGL(k_sfc_keys, dim3(grid_for(nd, B)), dim3(B), 0, s, cap_dst.p, nd, key.p, idx.p);
+        sort_pairs(T, key.p, key_sorted.p, idx.p, qorder.p, nd, 2 * kSfcOrder + 3, s);
         DevBuf<int64_t> cand_cnt(nd + 1, s);
         CR_HIP(hipMemsetAsync(cand_cnt.p + nd, 0, sizeof(int64_t), s));
         hipLaunchKernelGGL(k_search<false>, dim3(qblocks), dim3(64), 0, s, cap_dst.p, qorder.p, nd, nodes.p,

@@ -44,23 +44,46 @@ CR_HD float4 cell_cap(const double *p, int nv) {
     return pad_cap(m, r);
 }
 
-CR_HD uint32_t spread3(uint32_t v) {
-    v &= 0x3ffu;
-    v = (v | (v << 16)) & 0x030000ffu;
-    v = (v | (v << 8)) & 0x0300f00fu;
-    v = (v | (v << 4)) & 0x030c30c3u;
-    v = (v | (v << 2)) & 0x09249249u;
-    return v;
+// Space-filling-curve key of a cap centre: the cube face the centre projects
+// to (3 bits) and the Hilbert index of its gnomonic (u, v) on that face at
+// 2^kSfcOrder x 2^kSfcOrder resolution (28 bits).  Runs of consecutive keys
+// are compact patches of the sphere (no long jumps inside a face), which keeps
+// tree nodes of 8 consecutive cells and search clusters of 64 cells small.
+constexpr int kSfcOrder = 14;
+
+CR_HD uint32_t hilbert_d(uint32_t x, uint32_t y) {
+    uint32_t d = 0;
+    for (uint32_t s = 1u << (kSfcOrder - 1); s > 0; s >>= 1) {
+        const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+        d += s * s * ((3u * rx) ^ ry);
+        if (ry == 0) {  // rotate the quadrant
+            if (rx == 1) {
+                x = s - 1 - (x & (s - 1)) + (x & ~(s - 1));
+                y = s - 1 - (y & (s - 1)) + (y & ~(s - 1));
+            }
+            const uint32_t t = x;
+            x = y;
+            y = t;
+        }
+    }
+    return d;
 }
 
-CR_HD uint32_t quantize10(float x) {
-    int v = (int)((x + 1.0f) * 512.0f);
-    return (uint32_t)(v < 0 ? 0 : (v > 1023 ? 1023 : v));
-}
-
-// 30-bit Morton key of a cap centre.
-CR_HD uint32_t morton_key(const float4 &q) {
-    return spread3(quantize10(q.x)) | (spread3(quantize10(q.y)) << 1) | (spread3(quantize10(q.z)) << 2);
+CR_HD uint32_t sfc_key(const float4 &q) {
+    const float ax = fabsf(q.x), ay = fabsf(q.y), az = fabsf(q.z);
+    int f;
+    float m, u, v;
+    if (ax >= ay && ax >= az) { f = 0; m = q.x; u = q.y; v = q.z; }
+    else if (ay >= az) { f = 1; m = q.y; u = q.z; v = q.x; }
+    else { f = 2; m = q.z; u = q.x; v = q.y; }
+    const uint32_t face = (uint32_t)(2 * f + (m < 0.0f ? 1 : 0));
+    const float am = fabsf(m) > 0.0f ? fabsf(m) : 1.0f;
+    const float n = (float)(1u << kSfcOrder);
+    int iu = (int)((u / am + 1.0f) * 0.5f * n), iv = (int)((v / am + 1.0f) * 0.5f * n);
+    const int top = (1 << kSfcOrder) - 1;
+    iu = iu < 0 ? 0 : (iu > top ? top : iu);
+    iv = iv < 0 ? 0 : (iv > top ? top : iv);
+    return (face << (2 * kSfcOrder)) | hilbert_d((uint32_t)iu, (uint32_t)iv);
 }
 
 // Parent cap of children [c0, c1): ball around the normalised mean of the child
