@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "regridder.hpp"
@@ -142,24 +143,35 @@ cr_status cr_regridder_copy_csr(const cr_regridder *R, int32_t transpose, int64_
             CR_HIP(hipMemcpy(c.data(), A.col.p, A.col.bytes(), hipMemcpyDeviceToHost));
             CR_HIP(hipMemcpy(v.data(), A.val.p, A.val.bytes(), hipMemcpyDeviceToHost));
         }
-        std::vector<int32_t> order;
-        for (int64_t i = 0; i < A.nrows; ++i) {
-            const int64_t k0 = rp[i], n = rp[i + 1] - rp[i];
-            bool sorted = true;
-            for (int64_t k = 1; k < n && sorted; ++k) sorted = c[k0 + k - 1] < c[k0 + k];
-            if (sorted) {
-                if (col) std::copy(c.begin() + k0, c.begin() + k0 + n, col + k0);
-                if (val) std::copy(v.begin() + k0, v.begin() + k0 + n, val + k0);
-                continue;
+        auto sort_rows = [&](int64_t i0, int64_t i1) {
+            std::vector<int32_t> order;
+            for (int64_t i = i0; i < i1; ++i) {
+                const int64_t k0 = rp[i], n = rp[i + 1] - rp[i];
+                bool sorted = true;
+                for (int64_t k = 1; k < n && sorted; ++k) sorted = c[k0 + k - 1] < c[k0 + k];
+                if (sorted) {
+                    if (col) std::copy(c.begin() + k0, c.begin() + k0 + n, col + k0);
+                    if (val) std::copy(v.begin() + k0, v.begin() + k0 + n, val + k0);
+                    continue;
+                }
+                order.resize(n);
+                for (int64_t k = 0; k < n; ++k) order[k] = (int32_t)k;
+                std::sort(order.begin(), order.end(),
+                          [&](int32_t a, int32_t b) { return c[k0 + a] < c[k0 + b]; });
+                for (int64_t k = 0; k < n; ++k) {
+                    if (col) col[k0 + k] = c[k0 + order[k]];
+                    if (val) val[k0 + k] = v[k0 + order[k]];
+                }
             }
-            order.resize(n);
-            for (int64_t k = 0; k < n; ++k) order[k] = (int32_t)k;
-            std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return c[k0 + a] < c[k0 + b]; });
-            for (int64_t k = 0; k < n; ++k) {
-                if (col) col[k0 + k] = c[k0 + order[k]];
-                if (val) val[k0 + k] = v[k0 + order[k]];
-            }
+        };
+        const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(16, A.nrows / 65536));
+        std::vector<std::thread> th;
+        const int64_t per = (A.nrows + nth - 1) / nth;
+        for (int t = 0; t < nth; ++t) {
+            const int64_t i0 = t * per, i1 = std::min(A.nrows, i0 + per);
+            if (i0 < i1) th.emplace_back(sort_rows, i0, i1);
         }
+        for (auto &x : th) x.join();
         if (rowptr) std::copy(rp.begin(), rp.end(), rowptr);
     });
 }

@@ -7,10 +7,12 @@
 //
 //   * rows are cut into tiles (contiguous rows, at most kTileRows rows,
 //     kTileCols distinct columns and kTileThreads * kTileEnt non-zeros);
-//   * a tile's non-zeros are dealt out evenly to the threads of its workgroup
-//     (merge-path split: thread t takes entries [t*E, (t+1)*E), cutting rows
-//     anywhere), so long and short rows cost the same per thread;
-//   * each thread keeps its E entries (packed local column + segment id, and
+//   * a tile's rows are cut into pieces of at most 8 non-zeros, sorted by
+//     length and dealt out to the workgroup's threads in rounds, each wave's
+//     64 pieces of a round padded to the longest (tileplan.hpp): long and
+//     short rows cost the same per thread, and every lane of a wave ends its
+//     segments on the same slot (a wave-uniform mask, no divergent branch);
+//   * each thread keeps its slots (packed local column + segment id, and
 //     the value) in registers for all fields of the launch;
 //   * per field, the tile's distinct columns of x are loaded once (ascending
 //     column order: runs of neighbouring cells, coalesced) into LDS, every
@@ -28,6 +30,7 @@
 // scaled by the FP64 reciprocal area; FP64 fields use fused multiply-adds and a
 // true division by the area.
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "regridder.hpp"
@@ -63,6 +66,17 @@ struct SegAcc<float> {
     __device__ float value() const { return (float)((double)s - (double)c); }
 };
 
+// FP32 fields summed in FP64: the FP64 A value times the (exactly widened)
+// FP32 field value, fused multiply-add in FP64.  On gfx950 an FP64 FMA issues
+// at the rate of an unpacked FP32 FMA, so this costs 2 VALU operations per
+// product (convert + FMA) against 4 for a Kahan step, with smaller error.
+struct SegAcc64 {
+    double s;
+    __device__ void init() { s = 0.0; }
+    __device__ void add(double v, float x) { s = fma(v, (double)x, s); }
+    __device__ float value() const { return (float)s; }
+};
+
 // scale = 1/area (FP32 fields) or area (FP64 fields), nullptr: no normalisation
 __device__ __forceinline__ float row_finish(double acc, double scale, bool has, float *) {
     return (float)(has ? acc * scale : acc);
@@ -78,13 +92,31 @@ struct alignas(sizeof(T) * F >= 16 ? 16 : sizeof(T) * F) Pack {
     T v[F];
 };
 
-template <class T, int F>
+// MODE 0: T accumulation (FP64 fields: FMA; FP32 fields: Kahan);
+// MODE 1: FP32 fields accumulated in FP64 (values from the FP64 A).
+// OFF32: every cell offset (cell * stride) fits 32 bits, so loads and stores
+// use a per-field base address plus a 32-bit offset.
+template <int MODE, class T>
+struct TileAcc {
+    using type = SegAcc<T>;
+    using val = T;
+};
+template <>
+struct TileAcc<1, float> {
+    using type = SegAcc64;
+    using val = double;
+};
+
+template <class T, int F, int MODE, bool OFF32>
 __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
     const TileDesc *__restrict__ tiles, int64_t ntiles, int64_t tiles_per_xcd, int nchunks,
-    int64_t chunk, const uint32_t *__restrict__ keys, const T *__restrict__ vals,
+    int64_t chunk, const uint32_t *__restrict__ keys, const typename TileAcc<MODE, T>::val *__restrict__ vals,
     const int32_t *__restrict__ cols, const uint16_t *__restrict__ rseg, const T *__restrict__ x,
     int64_t sc, int64_t sf, T *__restrict__ y, int64_t dc, int64_t df, int64_t nf,
     const double *__restrict__ scale) {
+    using Acc = typename TileAcc<MODE, T>::type;
+    using V = typename TileAcc<MODE, T>::val;
+    using Off = typename std::conditional<OFF32, uint32_t, int64_t>::type;
     __shared__ Pack<T, F> xs[kTileCols];
     __shared__ Pack<T, F> seg[kTileSegs + 1];
     const int64_t b = blockIdx.x;
@@ -99,40 +131,41 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
     const int E = d.ent;
 
     uint32_t key[kTileEnt];
-    T v[kTileEnt];
+    V v[kTileEnt];
 #pragma unroll
     for (int e = 0; e < kTileEnt; ++e) {
         key[e] = 0u;
-        v[e] = T(0);
+        v[e] = V(0);
         if (e < E) {
             const int64_t q = d.ent_off + (int64_t)e * kTileThreads + tid;
             key[e] = keys[q];
             v[e] = vals[q];
         }
     }
-    // flush[e]: entry e ends its segment (the next slot starts another one)
-    uint32_t flush = 0u;
-#pragma unroll
-    for (int e = 0; e < kTileEnt; ++e) {
-        const uint32_t nxt = (e + 1 < kTileEnt) ? (key[e + 1] >> 16) : 0xffffffffu;
-        if (e < E && (e + 1 == E || nxt != (key[e] >> 16))) flush |= 1u << e;
-    }
-    int64_t xo[kColsPerThread];
+    // bit e: slot e ends a segment for every lane of this wave (wave-uniform)
+    const uint32_t flush = __builtin_amdgcn_readfirstlane(d.flush[tid >> 6]);
+    Off xo[kColsPerThread];
+    bool xin[kColsPerThread];
 #pragma unroll
     for (int j = 0; j < kColsPerThread; ++j) {
         const int u = tid + j * kTileThreads;
-        xo[j] = u < d.ncols ? (int64_t)cols[d.col_off + u] * sc : -1;
+        xin[j] = u < d.ncols;
+        xo[j] = xin[j] ? (Off)((int64_t)cols[d.col_off + u] * sc) : (Off)0;
     }
     const uint16_t *rs = rseg + d.row0 + tile;
     int sb[kRowsPerThread], se[kRowsPerThread];
+    bool rin[kRowsPerThread];
+    Off yo[kRowsPerThread];
     double rscale[kRowsPerThread];
 #pragma unroll
     for (int i = 0; i < kRowsPerThread; ++i) {
         const int r = tid + i * kTileThreads;
+        rin[i] = r < d.nrows;
         sb[i] = 0;
         se[i] = 0;
         rscale[i] = 1.0;
-        if (r < d.nrows) {
+        yo[i] = rin[i] ? (Off)((d.row0 + r) * dc) : (Off)0;
+        if (rin[i]) {
             sb[i] = rs[r];
             se[i] = rs[r + 1];
             if (scale) rscale[i] = scale[d.row0 + r];
@@ -141,10 +174,12 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
 
     T pre[F][kColsPerThread];
 #pragma unroll
-    for (int f = 0; f < F; ++f)
+    for (int f = 0; f < F; ++f) {
+        const T *xb = x + (fb + f) * sf;  // uniform per field
+        const bool fok = fb + f < fe;
 #pragma unroll
-        for (int j = 0; j < kColsPerThread; ++j)
-            pre[f][j] = (xo[j] >= 0 && fb + f < fe) ? x[xo[j] + (fb + f) * sf] : T(0);
+        for (int j = 0; j < kColsPerThread; ++j) pre[f][j] = (xin[j] && fok) ? xb[xo[j]] : T(0);
+    }
 
     for (int64_t f0 = fb; f0 < fe; f0 += F) {
 #pragma unroll
@@ -157,15 +192,17 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
         const int64_t fn = f0 + F;
         if (fn < fe) {
 #pragma unroll
-            for (int f = 0; f < F; ++f)
+            for (int f = 0; f < F; ++f) {
+                const T *xb = x + (fn + f) * sf;
+                const bool fok = fn + f < fe;
 #pragma unroll
-                for (int j = 0; j < kColsPerThread; ++j)
-                    pre[f][j] = (xo[j] >= 0 && fn + f < fe) ? x[xo[j] + (fn + f) * sf] : T(0);
+                for (int j = 0; j < kColsPerThread; ++j) pre[f][j] = (xin[j] && fok) ? xb[xo[j]] : T(0);
+            }
         }
         __syncthreads();
         // segment sums for F fields at once (padding slots: value 0, sink segment)
         {
-            SegAcc<T> a[F];
+            Acc a[F];
 #pragma unroll
             for (int f = 0; f < F; ++f) a[f].init();
 #pragma unroll
@@ -190,9 +227,7 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
         // rows: add their segments in order, store coalesced
 #pragma unroll
         for (int i = 0; i < kRowsPerThread; ++i) {
-            const int r = tid + i * kTileThreads;
-            if (r < d.nrows) {
-                const int64_t row = d.row0 + r;
+            if (rin[i]) {
                 double acc[F];
 #pragma unroll
                 for (int f = 0; f < F; ++f) acc[f] = 0.0;
@@ -202,9 +237,12 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
                     for (int f = 0; f < F; ++f) acc[f] += (double)sp.v[f];
                 }
 #pragma unroll
-                for (int f = 0; f < F; ++f)
-                    if (f0 + f < fe)
-                        y[row * dc + (f0 + f) * df] = row_finish(acc[f], rscale[i], scale != nullptr, (T *)nullptr);
+                for (int f = 0; f < F; ++f) {
+                    if (f0 + f < fe) {
+                        T *yb = y + (f0 + f) * df;  // uniform per field
+                        yb[yo[i]] = row_finish(acc[f], rscale[i], scale != nullptr, (T *)nullptr);
+                    }
+                }
             }
         }
         // the next iteration's LDS writes of xs follow the barrier above (all
@@ -261,21 +299,29 @@ void drop_tile_plan(const Csr &A) {
 }
 
 // ---- launch --------------------------------------------------------------------------
-template <class T, int F>
+template <class T, int F, int MODE>
 static void launch_f(const TilePlan &P, int nchunks, const T *x, int64_t sc, int64_t sf, T *y,
-                     int64_t dc, int64_t df, int64_t nf, const double *scale, hipStream_t s) {
+                     int64_t dc, int64_t df, int64_t nf, const double *scale, int64_t nrows, int64_t ncols,
+                     hipStream_t s) {
     const int64_t tiles_per_xcd = (P.ntiles + 7) / 8;
     int64_t chunk = (nf + nchunks - 1) / nchunks;
     chunk = (chunk + F - 1) / F * F;
     nchunks = (int)((nf + chunk - 1) / chunk);
     const int64_t blocks = tiles_per_xcd * 8 * nchunks;
     CR_REQUIRE(blocks < (int64_t)1 << 31, CR_UNSUPPORTED, "too many tiles x field chunks for one launch");
-    const T *vals;
-    if constexpr (sizeof(T) == 4) vals = (const T *)P.val32.p;
-    else vals = (const T *)P.val64.p;
-    hipLaunchKernelGGL((k_spmm_tiles<T, F>), dim3((unsigned)blocks), dim3(kTileThreads), 0, s, P.tiles.p,
-                       P.ntiles, tiles_per_xcd, nchunks, chunk, P.key.p, vals, P.cols.p, P.rseg.p, x, sc, sf,
-                       y, dc, df, nf, scale);
+    using V = typename TileAcc<MODE, T>::val;
+    const V *vals;
+    if constexpr (sizeof(V) == 4) vals = (const V *)P.val32.p;
+    else vals = (const V *)P.val64.p;
+    const bool off32 = (ncols - 1) * sc < ((int64_t)1 << 32) && (nrows - 1) * dc < ((int64_t)1 << 32);
+    if (off32)
+        hipLaunchKernelGGL((k_spmm_tiles<T, F, MODE, true>), dim3((unsigned)blocks), dim3(kTileThreads), 0, s,
+                           P.tiles.p, P.ntiles, tiles_per_xcd, nchunks, chunk, P.key.p, vals, P.cols.p, P.rseg.p,
+                           x, sc, sf, y, dc, df, nf, scale);
+    else
+        hipLaunchKernelGGL((k_spmm_tiles<T, F, MODE, false>), dim3((unsigned)blocks), dim3(kTileThreads), 0, s,
+                           P.tiles.p, P.ntiles, tiles_per_xcd, nchunks, chunk, P.key.p, vals, P.cols.p, P.rseg.p,
+                           x, sc, sf, y, dc, df, nf, scale);
 }
 
 // Workgroups for a tile launch: enough (tile, chunk) pairs to keep every CU
@@ -287,25 +333,40 @@ int default_chunks(const TilePlan &P, int64_t nf, int F) {
     return (int)c;
 }
 
+// F: fields per LDS stage; acc64: FP32 fields accumulated in FP64 (else Kahan).
 template <class T>
-void launch_tiles(const TilePlan &P, int F, int nchunks, const T *x, int64_t sc, int64_t sf, T *y,
-                  int64_t dc, int64_t df, int64_t nf, const double *scale, hipStream_t s) {
+void launch_tiles(const TilePlan &P, int F, bool acc64, int nchunks, const T *x, int64_t sc, int64_t sf, T *y,
+                  int64_t dc, int64_t df, int64_t nf, const double *scale, int64_t nrows, int64_t ncols,
+                  hipStream_t s) {
     if (P.ntiles == 0) return;
     if (nchunks <= 0) nchunks = default_chunks(P, nf, F);
-    switch (F) {
-        case 1: launch_f<T, 1>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, s); break;
-        case 2: launch_f<T, 2>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, s); break;
-        case 4: launch_f<T, 4>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, s); break;
-        default:
-            if constexpr (sizeof(T) == 4) launch_f<T, 8>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, s);
-            else launch_f<T, 4>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, s);
-            break;
+    if constexpr (sizeof(T) == 4) {
+        if (acc64) {
+            switch (F) {
+                case 2: launch_f<T, 2, 1>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
+                case 8: launch_f<T, 8, 1>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
+                default: launch_f<T, 4, 1>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
+            }
+        } else {
+            switch (F) {
+                case 2: launch_f<T, 2, 0>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
+                case 8: launch_f<T, 8, 0>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
+                default: launch_f<T, 4, 0>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
+            }
+        }
+    } else {
+        (void)acc64;
+        switch (F) {
+            case 1: launch_f<T, 1, 0>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
+            case 2: launch_f<T, 2, 0>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
+            default: launch_f<T, 4, 0>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
+        }
     }
 }
 
-template void launch_tiles<float>(const TilePlan &, int, int, const float *, int64_t, int64_t, float *,
-                                  int64_t, int64_t, int64_t, const double *, hipStream_t);
-template void launch_tiles<double>(const TilePlan &, int, int, const double *, int64_t, int64_t, double *,
-                                   int64_t, int64_t, int64_t, const double *, hipStream_t);
+template void launch_tiles<float>(const TilePlan &, int, bool, int, const float *, int64_t, int64_t, float *,
+                                  int64_t, int64_t, int64_t, const double *, int64_t, int64_t, hipStream_t);
+template void launch_tiles<double>(const TilePlan &, int, bool, int, const double *, int64_t, int64_t, double *,
+                                   int64_t, int64_t, int64_t, const double *, int64_t, int64_t, hipStream_t);
 
 }  // namespace cr
