@@ -153,6 +153,22 @@ def main():
     torch.cuda.synchronize()
 
     # ---- batched FP32 apply (the step) ----------------------------------------
+    # every FP32 kernel variant once (5 launches each, median), then the autotuner's pick
+    variant_ms = {}
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    for v in range(cr.regrid_variants("float32")):
+        cr.regrid_set_variant(R, v)
+        fn = cr.regrid_plan(Y, R, X, dims=1)
+        ts = []
+        for _ in range(5):
+            ev[0].record()
+            fn()
+            ev[1].record()
+            torch.cuda.synchronize()
+            ts.append(ev[0].elapsed_time(ev[1]))
+        variant_ms[v] = round(float(np.median(ts)), 4)
+    cr.regrid_set_variant(R, -1)
+    log("[bench] FP32 variants (ms): " + ", ".join(f"{k}: {v:.3f}" for k, v in variant_ms.items()))
     variant, tune_ms = cr.regrid_tune(Y, R, X, dims=1)  # picks the kernel variant once
     log(f"[bench] autotuned cells-fastest variant {variant} ({tune_ms:.4f} ms)")
     launch = cr.regrid_plan(Y, R, X, dims=1)
@@ -302,6 +318,7 @@ def main():
             "workload": "0.25deg lon-lat 1440x720 -> HEALPix nside 256 nested (16-gon cells); "
                         "256 FP32 fields [n,64,4] cells fastest, dims=1",
             "fields": K, "n_src": n_src, "n_dst": n_dst, "nnz": nnz, "apply_variant": variant,
+            "variant_ms": variant_ms,
         },
         "build": {"pairs_per_s": round(nnz / build_s, 1), "ms": round(build_s * 1e3, 3), "nnz": nnz,
                   "candidates": ncand, "candidate_pairs_per_s": round(ncand / build_s, 1),

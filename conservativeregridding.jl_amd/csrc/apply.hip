@@ -256,17 +256,18 @@ static void launch_rows(const Csr &A, const T *x, int64_t sc, int64_t sf, T *y, 
 // Staged tile kernel (tiles.hip) followed by the rows it leaves to the long-row
 // kernel.  scale: 1/area for FP32 fields, area for FP64 fields, or nullptr.
 template <class T>
-void launch_tiles(const TilePlan &P, int F, bool acc64, int nchunks, const T *x, int64_t sc, int64_t sf, T *y,
-                  int64_t dc, int64_t df, int64_t nf, const double *scale, int64_t nrows, int64_t ncols,
-                  hipStream_t s);
+void launch_tiles(const TilePlan &P, int F, bool acc64, int nchunks, int chunk_major, const T *x, int64_t sc,
+                  int64_t sf, T *y, int64_t dc, int64_t df, int64_t nf, const double *scale, int64_t nrows,
+                  int64_t ncols, hipStream_t s);
 
 template <class T>
-static void launch_tiled(const Csr &A, int F, bool acc64, int nchunks, const T *x, int64_t sc, int64_t sf, T *y,
-                         int64_t dc, int64_t df, int64_t nf, const double *area, const double *inv,
-                         hipStream_t s) {
-    const TilePlan &P = tile_plan(A, s);
+static void launch_tiled(const Csr &A, int F, bool acc64, int nchunks, int chunk_major, const T *x, int64_t sc,
+                         int64_t sf, T *y, int64_t dc, int64_t df, int64_t nf, const double *area,
+                         const double *inv, hipStream_t s) {
     constexpr bool f32 = sizeof(T) == 4;
-    launch_tiles<T>(P, F, acc64, nchunks, x, sc, sf, y, dc, df, nf, f32 ? inv : area, A.nrows, A.ncols, s);
+    const TilePlan &P = tile_plan(A, f32 ? kRounds : kEven, s);
+    launch_tiles<T>(P, F, acc64, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, f32 ? inv : area, A.nrows,
+                    A.ncols, s);
     if (P.nhuge) {
         constexpr int FB = f32 ? 8 : 1;
         const int64_t gy = (nf + FB - 1) / FB;
@@ -281,21 +282,25 @@ static void launch_tiled(const Csr &A, int F, bool acc64, int nchunks, const T *
 // fields per LDS stage and Kahan sums, then the tile kernel with FP64 sums.
 struct F32Variant {
     bool tiled;
-    int width;   // FB (row kernel) or F (tile kernel)
-    bool acc64;  // tile kernel: FP64 accumulation
+    int width;        // FB (row kernel) or F (tile kernel)
+    bool acc64;       // tile kernel: FP64 accumulation
+    int chunks;       // tile kernel: field chunks per launch (0: enough workgroups)
+    int chunk_major;  // tile kernel: sweep one field chunk at a time
 };
-constexpr F32Variant kF32Variants[] = {{false, 1, false}, {false, 2, false}, {false, 4, false},
-                                       {false, 8, false}, {false, 16, false}, {true, 2, false},
-                                       {true, 4, false},  {true, 8, false},  {true, 2, true},
-                                       {true, 4, true},   {true, 8, true}};
+constexpr F32Variant kF32Variants[] = {
+    {false, 1, false, 0, 0}, {false, 2, false, 0, 0}, {false, 4, false, 0, 0}, {false, 8, false, 0, 0},
+    {false, 16, false, 0, 0}, {true, 2, false, 0, 0}, {true, 4, false, 0, 0},  {true, 8, false, 0, 0},
+    {true, 2, true, 0, 0},   {true, 4, true, 0, 0},   {true, 8, true, 0, 0},   {true, 4, true, 4, 0},
+    {true, 4, true, 8, 1},   {true, 4, true, 16, 1},  {true, 4, true, 1, 0}};
 constexpr int kNumVariants = sizeof(kF32Variants) / sizeof(kF32Variants[0]);
-constexpr int kDefaultF32Variant = 6;  // tiles, 4 fields per stage, Kahan
+constexpr int kDefaultF32Variant = 9;  // tiles, 4 fields per stage, FP64 sums
 
 static void launch_f32_variant(int v, const Csr &A, const float *x, int64_t sc, int64_t sf, float *y,
                                int64_t dc, int64_t df, int64_t nf, const double *inv, hipStream_t s) {
     const F32Variant &V = kF32Variants[v];
     if (V.tiled) {
-        launch_tiled<float>(A, V.width, V.acc64, 0, x, sc, sf, y, dc, df, nf, nullptr, inv, s);
+        launch_tiled<float>(A, V.width, V.acc64, V.chunks, V.chunk_major, x, sc, sf, y, dc, df, nf, nullptr,
+                            inv, s);
         return;
     }
     switch (V.width) {
@@ -325,7 +330,8 @@ void apply(const cr_regridder *R, bool transpose, int dtype, int64_t nf, const v
         double *y = (double *)dst;
         const double *ar = normalize ? area : nullptr;
         const int v = A.f64_variant >= 0 ? A.f64_variant : kDefaultF64Variant;
-        if (v == 1) launch_tiled<double>(A, nf >= 2 ? 2 : 1, false, 0, x, sc, sf, y, dc, df, nf, ar, nullptr, s);
+        if (v == 1)
+            launch_tiled<double>(A, nf >= 2 ? 2 : 1, false, 0, 0, x, sc, sf, y, dc, df, nf, ar, nullptr, s);
         else if (nf == 1) launch_rows<double, 1>(A, x, sc, sf, y, dc, df, nf, ar, nullptr, s);
         else launch_rows<double, 8>(A, x, sc, sf, y, dc, df, nf, ar, nullptr, s);
     } else if (sf == 1 && df == 1 && nf >= 64 && force_variant < 0) {

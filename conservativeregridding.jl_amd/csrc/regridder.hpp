@@ -41,14 +41,15 @@ struct Csr {
     int f64_variant = -1;      // FP64 kernel variant; -1: default
     // Kernel plans, made on first use (guarded: handles are shared by callers):
     // the staged-apply tiles, and the rows too long for the row kernels.
-    mutable std::shared_ptr<TilePlan> tiles;
+    mutable std::shared_ptr<TilePlan> tiles[2];  // per TileStyle
     mutable DevBuf<int32_t> long_rows;
     mutable int64_t nlong = -1;  // -1: not planned yet
     mutable std::mutex plan_mu;
 };
 
-// The tile plan of A, built on the first call (synchronises stream s once).
-const TilePlan &tile_plan(const Csr &A, hipStream_t s);
+// The tile plan of A in the given TileStyle, built on the first call
+// (synchronises stream s once).
+const TilePlan &tile_plan(const Csr &A, int style, hipStream_t s);
 // Forget the plan (after A's values change).
 void drop_tile_plan(const Csr &A);
 

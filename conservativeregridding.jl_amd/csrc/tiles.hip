@@ -107,10 +107,13 @@ struct TileAcc<1, float> {
     using val = double;
 };
 
-template <class T, int F, int MODE, bool OFF32>
+// UNI: segment ends are the plan's wave-uniform masks (kRounds plans); else
+// each lane finds them from its own slots (kEven plans).
+template <class T, int F, int MODE, bool OFF32, bool UNI>
 __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
     const TileDesc *__restrict__ tiles, int64_t ntiles, int64_t tiles_per_xcd, int nchunks,
-    int64_t chunk, const uint32_t *__restrict__ keys, const typename TileAcc<MODE, T>::val *__restrict__ vals,
+    int64_t chunk, int chunk_major, const uint32_t *__restrict__ keys,
+    const typename TileAcc<MODE, T>::val *__restrict__ vals,
     const int32_t *__restrict__ cols, const uint16_t *__restrict__ rseg, const T *__restrict__ x,
     int64_t sc, int64_t sf, T *__restrict__ y, int64_t dc, int64_t df, int64_t nf,
     const double *__restrict__ scale) {
@@ -121,9 +124,13 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
     __shared__ Pack<T, F> seg[kTileSegs + 1];
     const int64_t b = blockIdx.x;
     const int64_t local = b >> 3;
-    const int64_t tile = (b & 7) * tiles_per_xcd + local / nchunks;
+    // tile-major: the chunks of a tile run side by side; chunk-major: the whole
+    // GPU sweeps one field chunk at a time
+    const int64_t tl = chunk_major ? local % tiles_per_xcd : local / nchunks;
+    const int64_t ch = chunk_major ? local / tiles_per_xcd : local % nchunks;
+    const int64_t tile = (b & 7) * tiles_per_xcd + tl;
     if (tile >= ntiles) return;
-    const int64_t fb = (local % nchunks) * chunk;
+    const int64_t fb = ch * chunk;
     const int64_t fe = min(nf, fb + chunk);
     if (fb >= fe) return;
     const TileDesc d = tiles[tile];
@@ -142,8 +149,17 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
             v[e] = vals[q];
         }
     }
-    // bit e: slot e ends a segment for every lane of this wave (wave-uniform)
-    const uint32_t flush = __builtin_amdgcn_readfirstlane(d.flush[tid >> 6]);
+    // bit e: slot e ends a segment (for every lane of the wave when UNI)
+    uint32_t flush = 0u;
+    if constexpr (UNI) {
+        flush = __builtin_amdgcn_readfirstlane(d.flush[tid >> 6]);
+    } else {
+#pragma unroll
+        for (int e = 0; e < kTileEnt; ++e) {
+            const uint32_t nxt = (e + 1 < kTileEnt) ? (key[e + 1] >> 16) : 0xffffffffu;
+            if (e < E && (e + 1 == E || nxt != (key[e] >> 16))) flush |= 1u << e;
+        }
+    }
     Off xo[kColsPerThread];
     bool xin[kColsPerThread];
 #pragma unroll
@@ -252,9 +268,9 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
 }
 
 // ---- plan ----------------------------------------------------------------------------
-const TilePlan &tile_plan(const Csr &A, hipStream_t s) {
+const TilePlan &tile_plan(const Csr &A, int style, hipStream_t s) {
     std::lock_guard<std::mutex> lock(A.plan_mu);
-    if (A.tiles) return *A.tiles;
+    if (A.tiles[style]) return *A.tiles[style];
     auto P = std::make_shared<TilePlan>();
     const int64_t nrows = A.nrows, nnz = A.nnz;
     std::vector<int64_t> rp(nrows + 1);
@@ -266,7 +282,7 @@ const TilePlan &tile_plan(const Csr &A, hipStream_t s) {
         CR_HIP(hipMemcpyAsync(val.data(), A.val.p, A.val.bytes(), hipMemcpyDeviceToHost, s));
     }
     CR_HIP(hipStreamSynchronize(s));
-    HostTilePlan H = make_host_tile_plan(nrows, A.ncols, rp.data(), col.data(), val.data());
+    HostTilePlan H = make_host_tile_plan(style, nrows, A.ncols, rp.data(), col.data(), val.data());
 
     P->ntiles = (int64_t)H.desc.size();
     P->nslots = (int64_t)H.key.size();
@@ -289,20 +305,21 @@ const TilePlan &tile_plan(const Csr &A, hipStream_t s) {
     up(P->rseg.p, H.rseg.data(), H.rseg.size() * sizeof(uint16_t));
     up(P->huge.p, H.huge.data(), H.huge.size() * sizeof(int32_t));
     CR_HIP(hipStreamSynchronize(s));
-    A.tiles = P;
-    return *A.tiles;
+    A.tiles[style] = P;
+    return *A.tiles[style];
 }
 
 void drop_tile_plan(const Csr &A) {
     std::lock_guard<std::mutex> lock(A.plan_mu);
-    A.tiles.reset();
+    A.tiles[0].reset();
+    A.tiles[1].reset();
 }
 
 // ---- launch --------------------------------------------------------------------------
-template <class T, int F, int MODE>
-static void launch_f(const TilePlan &P, int nchunks, const T *x, int64_t sc, int64_t sf, T *y,
-                     int64_t dc, int64_t df, int64_t nf, const double *scale, int64_t nrows, int64_t ncols,
-                     hipStream_t s) {
+template <class T, int F, int MODE, bool UNI>
+static void launch_f(const TilePlan &P, int nchunks, int chunk_major, const T *x, int64_t sc, int64_t sf,
+                     T *y, int64_t dc, int64_t df, int64_t nf, const double *scale, int64_t nrows,
+                     int64_t ncols, hipStream_t s) {
     const int64_t tiles_per_xcd = (P.ntiles + 7) / 8;
     int64_t chunk = (nf + nchunks - 1) / nchunks;
     chunk = (chunk + F - 1) / F * F;
@@ -315,12 +332,14 @@ static void launch_f(const TilePlan &P, int nchunks, const T *x, int64_t sc, int
     else vals = (const V *)P.val64.p;
     const bool off32 = (ncols - 1) * sc < ((int64_t)1 << 32) && (nrows - 1) * dc < ((int64_t)1 << 32);
     if (off32)
-        hipLaunchKernelGGL((k_spmm_tiles<T, F, MODE, true>), dim3((unsigned)blocks), dim3(kTileThreads), 0, s,
-                           P.tiles.p, P.ntiles, tiles_per_xcd, nchunks, chunk, P.key.p, vals, P.cols.p, P.rseg.p,
+        hipLaunchKernelGGL((k_spmm_tiles<T, F, MODE, true, UNI>), dim3((unsigned)blocks), dim3(kTileThreads), 0, s,
+                           P.tiles.p, P.ntiles, tiles_per_xcd, nchunks, chunk, chunk_major, P.key.p, vals,
+                           P.cols.p, P.rseg.p,
                            x, sc, sf, y, dc, df, nf, scale);
     else
-        hipLaunchKernelGGL((k_spmm_tiles<T, F, MODE, false>), dim3((unsigned)blocks), dim3(kTileThreads), 0, s,
-                           P.tiles.p, P.ntiles, tiles_per_xcd, nchunks, chunk, P.key.p, vals, P.cols.p, P.rseg.p,
+        hipLaunchKernelGGL((k_spmm_tiles<T, F, MODE, false, UNI>), dim3((unsigned)blocks), dim3(kTileThreads), 0, s,
+                           P.tiles.p, P.ntiles, tiles_per_xcd, nchunks, chunk, chunk_major, P.key.p, vals,
+                           P.cols.p, P.rseg.p,
                            x, sc, sf, y, dc, df, nf, scale);
 }
 
@@ -334,39 +353,45 @@ int default_chunks(const TilePlan &P, int64_t nf, int F) {
 }
 
 // F: fields per LDS stage; acc64: FP32 fields accumulated in FP64 (else Kahan).
+// FP32 fields use kRounds plans (wave-uniform segment ends), FP64 fields kEven
+// plans (no padding; the same segments for any field count, so a batched FP64
+// apply equals the loop of single-field applies bit for bit).
 template <class T>
-void launch_tiles(const TilePlan &P, int F, bool acc64, int nchunks, const T *x, int64_t sc, int64_t sf, T *y,
-                  int64_t dc, int64_t df, int64_t nf, const double *scale, int64_t nrows, int64_t ncols,
-                  hipStream_t s) {
+void launch_tiles(const TilePlan &P, int F, bool acc64, int nchunks, int chunk_major, const T *x, int64_t sc,
+                  int64_t sf, T *y, int64_t dc, int64_t df, int64_t nf, const double *scale, int64_t nrows,
+                  int64_t ncols, hipStream_t s) {
     if (P.ntiles == 0) return;
     if (nchunks <= 0) nchunks = default_chunks(P, nf, F);
+    nchunks = (int)std::max<int64_t>(1, std::min<int64_t>(nchunks, (nf + F - 1) / F));
     if constexpr (sizeof(T) == 4) {
         if (acc64) {
             switch (F) {
-                case 2: launch_f<T, 2, 1>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
-                case 8: launch_f<T, 8, 1>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
-                default: launch_f<T, 4, 1>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
+                case 2: launch_f<T, 2, 1, true>(P, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
+                case 8: launch_f<T, 8, 1, true>(P, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
+                default: launch_f<T, 4, 1, true>(P, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
             }
         } else {
             switch (F) {
-                case 2: launch_f<T, 2, 0>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
-                case 8: launch_f<T, 8, 0>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
-                default: launch_f<T, 4, 0>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
+                case 2: launch_f<T, 2, 0, true>(P, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
+                case 8: launch_f<T, 8, 0, true>(P, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
+                default: launch_f<T, 4, 0, true>(P, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
             }
         }
     } else {
         (void)acc64;
         switch (F) {
-            case 1: launch_f<T, 1, 0>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
-            case 2: launch_f<T, 2, 0>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
-            default: launch_f<T, 4, 0>(P, nchunks, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
+            case 1: launch_f<T, 1, 0, false>(P, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
+            case 2: launch_f<T, 2, 0, false>(P, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
+            default: launch_f<T, 4, 0, false>(P, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
         }
     }
 }
 
-template void launch_tiles<float>(const TilePlan &, int, bool, int, const float *, int64_t, int64_t, float *,
-                                  int64_t, int64_t, int64_t, const double *, int64_t, int64_t, hipStream_t);
-template void launch_tiles<double>(const TilePlan &, int, bool, int, const double *, int64_t, int64_t, double *,
-                                   int64_t, int64_t, int64_t, const double *, int64_t, int64_t, hipStream_t);
+template void launch_tiles<float>(const TilePlan &, int, bool, int, int, const float *, int64_t, int64_t,
+                                  float *, int64_t, int64_t, int64_t, const double *, int64_t, int64_t,
+                                  hipStream_t);
+template void launch_tiles<double>(const TilePlan &, int, bool, int, int, const double *, int64_t, int64_t,
+                                   double *, int64_t, int64_t, int64_t, const double *, int64_t, int64_t,
+                                   hipStream_t);
 
 }  // namespace cr

@@ -430,8 +430,9 @@ def regrid_variants(dtype="float32") -> int:
 def regrid_set_variant(R: Regridder, variant: int, dtype="float32") -> None:
     """Pin the apply kernel variant used for this direction of ``R`` and this
     field dtype (``-1`` restores the default).  FP32: 0-4 row kernel with 1-16
-    fields per thread, 5-7 staged tile kernel with 2/4/8 fields per stage;
-    FP64: 0 row kernel, 1 tile kernel."""
+    fields per thread, 5-7 staged tile kernel with 2/4/8 fields per stage and
+    Kahan sums, 8-10 the same with FP64 sums (default 9), 11-14 variant 9 with
+    other field chunkings; FP64: 0 row kernel, 1 tile kernel (default)."""
     L.check(L.lib.cr_regrid_set_variant(R._core.h, 1 if R._t else 0, _dtype_code(dtype), int(variant)),
             "regrid_set_variant")
 

@@ -207,9 +207,11 @@ cr_status cr_regrid_tune(cr_regridder *R, int32_t transpose, int64_t nfields, co
  * gives the number of variants for a dtype; cr_regrid_set_variant pins the
  * variant cr_regrid uses for one direction of R (-1 restores the default).
  *   CR_F32: 0-4 row kernel with 1, 2, 4, 8, 16 fields per thread; 5-7 staged
- *           tile kernel with 2, 4, 8 fields per LDS stage (default 6).  FP32
- *           fields with unit field strides and >= 64 fields always use the
- *           fields-fastest kernel.
+ *           tile kernel with 2, 4, 8 fields per LDS stage and FP32 Kahan sums;
+ *           8-10 the tile kernel with 2, 4, 8 fields per stage and FP64 sums
+ *           (default 9); 11-14 variant 9 with 4 field chunks, 8 and 16 chunks
+ *           swept one at a time, and a single chunk.  FP32 fields with unit
+ *           field strides and >= 64 fields always use the fields-fastest kernel.
  *   CR_F64: 0 row kernel, 1 staged tile kernel (default).
  */
 int32_t cr_regrid_variant_count(int32_t dtype);
