@@ -261,11 +261,11 @@ void launch_tiles(const TilePlan &P, int F, bool acc64, int nchunks, int chunk_m
                   int64_t ncols, hipStream_t s);
 
 template <class T>
-static void launch_tiled(const Csr &A, int F, bool acc64, int nchunks, int chunk_major, const T *x, int64_t sc,
-                         int64_t sf, T *y, int64_t dc, int64_t df, int64_t nf, const double *area,
+static void launch_tiled(const Csr &A, int geom, int F, bool acc64, int nchunks, int chunk_major, const T *x,
+                         int64_t sc, int64_t sf, T *y, int64_t dc, int64_t df, int64_t nf, const double *area,
                          const double *inv, hipStream_t s) {
     constexpr bool f32 = sizeof(T) == 4;
-    const TilePlan &P = tile_plan(A, f32 ? kRounds : kEven, s);
+    const TilePlan &P = tile_plan(A, f32 ? kRounds : kEven, geom, s);
     launch_tiles<T>(P, F, acc64, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, f32 ? inv : area, A.nrows,
                     A.ncols, s);
     if (P.nhuge) {
@@ -286,12 +286,15 @@ struct F32Variant {
     bool acc64;       // tile kernel: FP64 accumulation
     int chunks;       // tile kernel: field chunks per launch (0: enough workgroups)
     int chunk_major;  // tile kernel: sweep one field chunk at a time
+    int geom;         // tile kernel: TileGeomId
 };
 constexpr F32Variant kF32Variants[] = {
-    {false, 1, false, 0, 0}, {false, 2, false, 0, 0}, {false, 4, false, 0, 0}, {false, 8, false, 0, 0},
-    {false, 16, false, 0, 0}, {true, 2, false, 0, 0}, {true, 4, false, 0, 0},  {true, 8, false, 0, 0},
-    {true, 2, true, 0, 0},   {true, 4, true, 0, 0},   {true, 8, true, 0, 0},   {true, 4, true, 4, 0},
-    {true, 4, true, 8, 1},   {true, 4, true, 16, 1},  {true, 4, true, 1, 0}};
+    {false, 1, false, 0, 0, 0},  {false, 2, false, 0, 0, 0},  {false, 4, false, 0, 0, 0},
+    {false, 8, false, 0, 0, 0},  {false, 16, false, 0, 0, 0}, {true, 2, false, 0, 0, 0},
+    {true, 4, false, 0, 0, 0},   {true, 8, false, 0, 0, 0},   {true, 2, true, 0, 0, 0},
+    {true, 4, true, 0, 0, 0},    {true, 8, true, 0, 0, 0},    {true, 4, true, 4, 0, 0},
+    {true, 4, true, 8, 1, 0},    {true, 4, true, 16, 1, 0},   {true, 4, true, 1, 0, 0},
+    {true, 4, true, 0, 0, 1},    {true, 2, true, 0, 0, 1},    {true, 4, true, 4, 0, 1}};
 constexpr int kNumVariants = sizeof(kF32Variants) / sizeof(kF32Variants[0]);
 constexpr int kDefaultF32Variant = 9;  // tiles, 4 fields per stage, FP64 sums
 
@@ -299,8 +302,8 @@ static void launch_f32_variant(int v, const Csr &A, const float *x, int64_t sc, 
                                int64_t dc, int64_t df, int64_t nf, const double *inv, hipStream_t s) {
     const F32Variant &V = kF32Variants[v];
     if (V.tiled) {
-        launch_tiled<float>(A, V.width, V.acc64, V.chunks, V.chunk_major, x, sc, sf, y, dc, df, nf, nullptr,
-                            inv, s);
+        launch_tiled<float>(A, V.geom, V.width, V.acc64, V.chunks, V.chunk_major, x, sc, sf, y, dc, df, nf,
+                            nullptr, inv, s);
         return;
     }
     switch (V.width) {
@@ -331,7 +334,8 @@ void apply(const cr_regridder *R, bool transpose, int dtype, int64_t nf, const v
         const double *ar = normalize ? area : nullptr;
         const int v = A.f64_variant >= 0 ? A.f64_variant : kDefaultF64Variant;
         if (v == 1)
-            launch_tiled<double>(A, nf >= 2 ? 2 : 1, false, 0, 0, x, sc, sf, y, dc, df, nf, ar, nullptr, s);
+            launch_tiled<double>(A, kGeomS, nf >= 2 ? 2 : 1, false, 0, 0, x, sc, sf, y, dc, df, nf, ar, nullptr,
+                                 s);
         else if (nf == 1) launch_rows<double, 1>(A, x, sc, sf, y, dc, df, nf, ar, nullptr, s);
         else launch_rows<double, 8>(A, x, sc, sf, y, dc, df, nf, ar, nullptr, s);
     } else if (sf == 1 && df == 1 && nf >= 64 && force_variant < 0) {

@@ -19,6 +19,7 @@
 namespace cr {
 
 struct TilePlan {
+    int geom = kGeomS;
     int64_t ntiles = 0;
     DevBuf<TileDesc> tiles;
     DevBuf<uint32_t> key;    // per slot: local column (low 16 bits) | segment id (high 16)
@@ -41,15 +42,15 @@ struct Csr {
     int f64_variant = -1;      // FP64 kernel variant; -1: default
     // Kernel plans, made on first use (guarded: handles are shared by callers):
     // the staged-apply tiles, and the rows too long for the row kernels.
-    mutable std::shared_ptr<TilePlan> tiles[2];  // per TileStyle
+    mutable std::shared_ptr<TilePlan> tiles[2][2];  // per TileStyle and TileGeomId
     mutable DevBuf<int32_t> long_rows;
     mutable int64_t nlong = -1;  // -1: not planned yet
     mutable std::mutex plan_mu;
 };
 
-// The tile plan of A in the given TileStyle, built on the first call
-// (synchronises stream s once).
-const TilePlan &tile_plan(const Csr &A, int style, hipStream_t s);
+// The tile plan of A in the given TileStyle and TileGeomId, built on the first
+// call (synchronises stream s once).
+const TilePlan &tile_plan(const Csr &A, int style, int geom, hipStream_t s);
 // Forget the plan (after A's values change).
 void drop_tile_plan(const Csr &A);
 

@@ -1,20 +1,26 @@
 // tileplan.hpp -- row tiles of a CSR matrix for the staged apply (host code,
 // shared by the library and host-side checks).
 //
-// A tile is a contiguous range of rows with at most kTileRows rows and
-// kTileCols distinct columns.  Its rows are cut into pieces of at most
-// kPieceLen non-zeros (a row of n non-zeros gives ceil(n / kPieceLen) pieces
-// of near-equal length); a piece is a *segment*: the kernel sums it in one
-// accumulator and writes the sum to LDS slot `segment id`.  Pieces are sorted
-// by length (longest first) and dealt out in rounds of kTileThreads: round k
-// gives piece k*256 + t to thread t.  In every round, the 64 pieces of one
-// wave are padded to the longest of them, so all lanes of a wave end their
-// segments on the same entry: the end-of-segment test is a wave-uniform bit
-// mask (TileDesc::flush), not a per-lane branch.  A thread's entries over all
-// rounds (at most kTileEnt) stay in registers for the whole launch.
+// A tile is a contiguous range of rows with at most kRows rows and kCols
+// distinct columns (TileGeom).  Two layouts of its non-zeros over the
+// workgroup's threads:
 //
-// Rows with more than kTileCols non-zeros ("huge") stay in their tile with no
-// entries and are listed for another kernel.
+// kRounds: rows are cut into pieces of at most kPieceLen non-zeros (a row of
+// n non-zeros gives ceil(n / kPieceLen) pieces of near-equal length); a piece
+// is a *segment*: the kernel sums it in one accumulator and writes the sum to
+// LDS slot `segment id`.  Pieces are sorted by length (longest first) and
+// dealt out in rounds of kThreads: round k gives piece k*kThreads + t to thread
+// t.  In every round, the 64 pieces of one wave are padded to the longest of
+// them, so all lanes of a wave end their segments on the same slot: the
+// end-of-segment test is a wave-uniform bit mask (TileDesc::flush), not a
+// per-lane branch.
+//
+// kEven: the non-zeros (row-major) are dealt out in equal runs; see
+// make_even_tile_plan.
+//
+// Either way a thread's slots (at most kEnt) stay in registers for the whole
+// launch.  Rows with more than kCols non-zeros ("huge") stay in their tile
+// with no entries and are listed for another kernel.
 #pragma once
 
 #include <algorithm>
@@ -25,28 +31,38 @@
 
 namespace cr {
 
-constexpr int kTileThreads = 256;
-constexpr int kTileWaves = kTileThreads / 64;
-constexpr int kTileEnt = 16;
-constexpr int kTileCols = 1024;
-constexpr int kTileRows = 512;
+// Tile geometry: threads per workgroup, rows and distinct columns per tile,
+// segments (pieces) per tile; every thread holds at most kEnt entry slots.
+template <int THREADS, int ROWS, int COLS, int SEGS>
+struct TileGeom {
+    static constexpr int kThreads = THREADS;
+    static constexpr int kWaves = THREADS / 64;
+    static constexpr int kEnt = 16;
+    static constexpr int kCols = COLS;
+    static constexpr int kRows = ROWS;
+    static constexpr int kSegs = SEGS;
+    static constexpr int kColsPerThread = COLS / THREADS;
+    static constexpr int kRowsPerThread = ROWS / THREADS;
+    static_assert(COLS <= 65536 && SEGS < 65535, "local indices are 16 bits");
+    static_assert(kEnt <= 32, "flush masks are 32 bits");
+    static_assert(SEGS >= ROWS + THREADS, "an even split makes up to ROWS + THREADS segments");
+};
+using GeomS = TileGeom<256, 512, 1024, 1024>;   // 4 waves, 512-row tiles
+using GeomL = TileGeom<512, 1024, 2048, 1536>;  // 8 waves, 1024-row tiles
+enum TileGeomId { kGeomS = 0, kGeomL = 1 };
+constexpr int kMaxTileWaves = 8;
 constexpr int kPieceLen = 8;
-// pieces: at most one per row plus one per kPieceLen entries of a split row
-constexpr int kTileSegs = kTileRows + kTileThreads * kTileEnt / kPieceLen;
-constexpr int kColsPerThread = kTileCols / kTileThreads;
-constexpr int kRowsPerThread = kTileRows / kTileThreads;
-static_assert(kTileCols <= 65536 && kTileSegs < 65535, "local indices are 16 bits");
-static_assert(kTileEnt <= 32, "flush masks are 32 bits");
+constexpr int kRowAlign = 32;  // rows per 128-byte line of a FP32 field
 
 struct TileDesc {
     int64_t row0;     // first row
-    int64_t ent_off;  // first entry slot (slots are entry-major: e * kTileThreads + thread)
+    int64_t ent_off;  // first entry slot (slots are entry-major: e * kThreads + thread)
     int64_t col_off;  // first distinct column in the column list
     int32_t nrows;    // rows in the tile
     int32_t ncols;    // distinct columns
     int32_t ent;      // entry slots per thread
     int32_t nseg;     // segments (pieces); segment id nseg is the padding sink
-    uint32_t flush[kTileWaves];  // per wave: bit e set = slot e ends a segment
+    uint32_t flush[kMaxTileWaves];  // per wave: bit e set = slot e ends a segment
 };
 
 struct HostTilePlan {
@@ -68,16 +84,17 @@ inline int piece_len(int64_t n, int j) {
 
 // Slots per thread of a tile whose pieces have the given length histogram
 // (hist[L] = pieces of length L, 1 <= L <= kPieceLen): the pieces sorted
-// longest first, dealt in rounds of kTileThreads; per wave and round the slot
+// longest first, dealt in rounds of kThreads; per wave and round the slot
 // count is the longest piece of that wave's 64.  Returns the largest per-wave
 // total (the tile's slots per thread).
+template <class G>
 inline int tile_slots(const int *hist) {
     int npieces = 0;
     for (int L = 1; L <= kPieceLen; ++L) npieces += hist[L];
     int worst = 0;
-    for (int w = 0; w < kTileWaves; ++w) {
+    for (int w = 0; w < G::kWaves; ++w) {
         int total = 0;
-        for (int start = w * 64; start < npieces; start += kTileThreads) {
+        for (int start = w * 64; start < npieces; start += G::kThreads) {
             // length of the piece at sorted position `start` (descending order)
             int pos = start, L = kPieceLen;
             while (L > 0 && pos >= hist[L]) pos -= hist[L--];
@@ -88,6 +105,7 @@ inline int tile_slots(const int *hist) {
     return worst;
 }
 
+template <class G>
 inline HostTilePlan make_rounds_tile_plan(int64_t nrows, int64_t ncols, const int64_t *rp,
                                           const int32_t *col, const double *val) {
     // pass 1: greedy tile boundaries (rows in order)
@@ -104,29 +122,38 @@ inline HostTilePlan make_rounds_tile_plan(int64_t nrows, int64_t ncols, const in
         Span cur{r, 0, 0};
         int hist[kPieceLen + 1] = {0};
         int npieces = 0;
-        while (r < nrows && cur.nrows < kTileRows) {
+        while (r < nrows && cur.nrows < G::kRows) {
             const int64_t k0 = rp[r], k1 = rp[r + 1], n = k1 - k0;
-            if (n > kTileCols) {  // a row alone over the column budget
+            if (n > G::kCols) {  // a row alone over the column budget
                 huge.push_back((int32_t)r);
                 ++cur.nrows;
                 ++r;
                 continue;
             }
             const int p = piece_count(n);
-            if (npieces + p > kTileSegs) break;
+            if (npieces + p > G::kSegs) break;
             int h2[kPieceLen + 1];
             std::copy(hist, hist + kPieceLen + 1, h2);
             for (int j = 0; j < p; ++j) ++h2[piece_len(n, j)];
-            if (tile_slots(h2) > kTileEnt) break;
+            if (tile_slots<G>(h2) > G::kEnt) break;
             int32_t nu = 0;
             for (int64_t k = k0; k < k1; ++k) nu += stamp[col[k]] != t;
-            if (cur.ncols + nu > kTileCols) break;
+            if (cur.ncols + nu > G::kCols) break;
             for (int64_t k = k0; k < k1; ++k) stamp[col[k]] = t;
             std::copy(h2, h2 + kPieceLen + 1, hist);
             npieces += p;
             cur.ncols += nu;
             ++cur.nrows;
             ++r;
+        }
+        // end a full tile on a multiple of kRowAlign rows when it can give rows
+        // back, so no 128-byte line of a FP32 output field is shared by two
+        // tiles (the column count stays an upper bound)
+        const int back = (int)(r % kRowAlign);
+        if (r < nrows && back && cur.nrows > back) {
+            r -= back;
+            cur.nrows -= back;
+            while (!huge.empty() && huge.back() >= r) huge.pop_back();  // listed again later
         }
         spans.push_back(cur);
     }
@@ -139,16 +166,16 @@ inline HostTilePlan make_rounds_tile_plan(int64_t nrows, int64_t ncols, const in
         int hist[kPieceLen + 1] = {0};
         for (int64_t rr = sp.row0; rr < sp.row0 + sp.nrows; ++rr) {
             const int64_t n = rp[rr + 1] - rp[rr];
-            if (n > kTileCols) continue;
+            if (n > G::kCols) continue;
             for (int j = 0; j < piece_count(n); ++j) ++hist[piece_len(n, j)];
         }
         TileDesc &d = desc[t];
         d.row0 = sp.row0;
         d.nrows = sp.nrows;
-        d.ent = tile_slots(hist);
+        d.ent = tile_slots<G>(hist);
         d.ent_off = ent_total;
         d.col_off = col_total;
-        ent_total += (int64_t)d.ent * kTileThreads;
+        ent_total += (int64_t)d.ent * G::kThreads;
         col_total += sp.ncols;  // capacity; the exact count is set in pass 2
     }
     std::vector<uint32_t> key(ent_total);
@@ -175,7 +202,7 @@ inline HostTilePlan make_rounds_tile_plan(int64_t nrows, int64_t ncols, const in
                 const int64_t rr = d.row0 + lr;
                 const int64_t n = rp[rr + 1] - rp[rr];
                 rs[lr] = (uint16_t)pieces.size();
-                if (n > kTileCols) continue;
+                if (n > G::kCols) continue;
                 for (int64_t k = rp[rr]; k < rp[rr + 1]; ++k) u.push_back(col[k]);
                 int64_t k = rp[rr];
                 for (int j = 0; j < piece_count(n); ++j) {
@@ -197,21 +224,21 @@ inline HostTilePlan make_rounds_tile_plan(int64_t nrows, int64_t ncols, const in
             std::stable_sort(order.begin(), order.end(),
                              [&](int32_t a, int32_t b) { return pieces[a].len > pieces[b].len; });
             // every slot starts as padding of the sink segment
-            for (int64_t q = 0; q < (int64_t)d.ent * kTileThreads; ++q) {
+            for (int64_t q = 0; q < (int64_t)d.ent * G::kThreads; ++q) {
                 key[d.ent_off + q] = (uint32_t)nseg << 16;
                 v32[d.ent_off + q] = 0.f;
                 v64[d.ent_off + q] = 0.0;
             }
-            for (int w = 0; w < kTileWaves; ++w) {
+            for (int w = 0; w < G::kWaves; ++w) {
                 uint32_t fm = 0u;
                 int off = 0;  // slots used by this wave's earlier rounds
-                for (int start = w * 64; start < nseg; start += kTileThreads) {
+                for (int start = w * 64; start < nseg; start += G::kThreads) {
                     const int L = pieces[order[start]].len;  // longest of this wave's 64
                     for (int l = 0; l < 64 && start + l < nseg; ++l) {
                         const Piece &pc = pieces[order[start + l]];
                         const int thr = w * 64 + l;
                         for (int e = 0; e < L; ++e) {
-                            const int64_t slot = d.ent_off + (int64_t)(off + e) * kTileThreads + thr;
+                            const int64_t slot = d.ent_off + (int64_t)(off + e) * G::kThreads + thr;
                             uint32_t lc = 0;
                             double vv = 0.0;
                             if (e < pc.len) {
@@ -259,6 +286,7 @@ inline HostTilePlan make_rounds_tile_plan(int64_t nrows, int64_t ncols, const in
 // its own run changes row (a per-lane test in the kernel: TileDesc::flush is
 // unused).  No padding beyond the last run: the cheaper layout when few fields
 // share the per-tile set-up.
+template <class G>
 inline HostTilePlan make_even_tile_plan(int64_t nrows, int64_t ncols, const int64_t *rp,
                                         const int32_t *col, const double *val) {
     // pass 1: greedy tile boundaries (rows in order)
@@ -269,14 +297,14 @@ inline HostTilePlan make_even_tile_plan(int64_t nrows, int64_t ncols, const int6
     std::vector<Span> spans;
     std::vector<int32_t> huge;
     std::vector<int32_t> stamp(ncols, -1);
-    const int64_t max_ent = (int64_t)kTileThreads * kTileEnt;
+    const int64_t max_ent = (int64_t)G::kThreads * G::kEnt;
     int64_t r = 0;
     while (r < nrows) {
         const int32_t t = (int32_t)spans.size();
         Span cur{r, 0, 0, 0};
-        while (r < nrows && cur.nrows < kTileRows) {
+        while (r < nrows && cur.nrows < G::kRows) {
             const int64_t k0 = rp[r], k1 = rp[r + 1], n = k1 - k0;
-            if (n > kTileCols) {  // a row alone over the column budget
+            if (n > G::kCols) {  // a row alone over the column budget
                 huge.push_back((int32_t)r);
                 ++cur.nrows;
                 ++r;
@@ -285,7 +313,7 @@ inline HostTilePlan make_even_tile_plan(int64_t nrows, int64_t ncols, const int6
             if (cur.nnz + n > max_ent) break;
             int32_t nu = 0;
             for (int64_t k = k0; k < k1; ++k) nu += stamp[col[k]] != t;
-            if (cur.ncols + nu > kTileCols) break;
+            if (cur.ncols + nu > G::kCols) break;
             for (int64_t k = k0; k < k1; ++k) stamp[col[k]] = t;
             cur.ncols += nu;
             cur.nnz += (int32_t)n;
@@ -302,11 +330,11 @@ inline HostTilePlan make_even_tile_plan(int64_t nrows, int64_t ncols, const int6
         TileDesc &d = desc[t];
         d.row0 = sp.row0;
         d.nrows = sp.nrows;
-        d.ent = (sp.nnz + kTileThreads - 1) / kTileThreads;
+        d.ent = (sp.nnz + G::kThreads - 1) / G::kThreads;
         d.ent_off = ent_total;
         d.col_off = col_total;
-        for (int w = 0; w < kTileWaves; ++w) d.flush[w] = 0u;
-        ent_total += (int64_t)d.ent * kTileThreads;
+        for (int w = 0; w < kMaxTileWaves; ++w) d.flush[w] = 0u;
+        ent_total += (int64_t)d.ent * G::kThreads;
         col_total += sp.ncols;  // capacity; the exact count is set in pass 2
     }
     std::vector<uint32_t> key(ent_total);
@@ -322,7 +350,7 @@ inline HostTilePlan make_even_tile_plan(int64_t nrows, int64_t ncols, const int6
             TileDesc &d = desc[t];
             u.clear();
             for (int64_t rr = d.row0; rr < d.row0 + d.nrows; ++rr)
-                if (rp[rr + 1] - rp[rr] <= kTileCols)
+                if (rp[rr + 1] - rp[rr] <= G::kCols)
                     for (int64_t k = rp[rr]; k < rp[rr + 1]; ++k) u.push_back(col[k]);
             std::sort(u.begin(), u.end());
             u.erase(std::unique(u.begin(), u.end()), u.end());
@@ -336,7 +364,7 @@ inline HostTilePlan make_even_tile_plan(int64_t nrows, int64_t ncols, const int6
             for (int32_t lr = 0; lr < d.nrows; ++lr) {
                 const int64_t rr = d.row0 + lr;
                 rs[lr] = (uint16_t)nseg;
-                if (rp[rr + 1] - rp[rr] > kTileCols) continue;
+                if (rp[rr + 1] - rp[rr] > G::kCols) continue;
                 bool first = true;
                 for (int64_t k = rp[rr]; k < rp[rr + 1]; ++k, ++q) {
                     const int64_t thr = q / E, e = q % E;
@@ -346,7 +374,7 @@ inline HostTilePlan make_even_tile_plan(int64_t nrows, int64_t ncols, const int6
                         prev_thread = thr;
                     }
                     const int32_t lc = (int32_t)(std::lower_bound(u.begin(), u.end(), col[k]) - u.begin());
-                    const int64_t slot = d.ent_off + e * kTileThreads + thr;
+                    const int64_t slot = d.ent_off + e * G::kThreads + thr;
                     key[slot] = (uint32_t)lc | ((uint32_t)cur << 16);
                     v32[slot] = (float)val[k];
                     v64[slot] = val[k];
@@ -354,9 +382,9 @@ inline HostTilePlan make_even_tile_plan(int64_t nrows, int64_t ncols, const int6
             }
             rs[d.nrows] = (uint16_t)nseg;
             d.nseg = nseg;
-            for (; q < E * kTileThreads; ++q) {  // padding: value 0, sink segment
+            for (; q < E * G::kThreads; ++q) {  // padding: value 0, sink segment
                 const int64_t thr = q / E, e = q % E;
-                const int64_t slot = d.ent_off + e * kTileThreads + thr;
+                const int64_t slot = d.ent_off + e * G::kThreads + thr;
                 key[slot] = (uint32_t)nseg << 16;
                 v32[slot] = 0.f;
                 v64[slot] = 0.0;
@@ -389,10 +417,11 @@ inline HostTilePlan make_even_tile_plan(int64_t nrows, int64_t ncols, const int6
 // (no padding, for one or two fields).
 enum TileStyle { kRounds = 0, kEven = 1 };
 
+template <class G>
 inline HostTilePlan make_host_tile_plan(int style, int64_t nrows, int64_t ncols, const int64_t *rp,
                                         const int32_t *col, const double *val) {
-    return style == kEven ? make_even_tile_plan(nrows, ncols, rp, col, val)
-                          : make_rounds_tile_plan(nrows, ncols, rp, col, val);
+    return style == kEven ? make_even_tile_plan<G>(nrows, ncols, rp, col, val)
+                          : make_rounds_tile_plan<G>(nrows, ncols, rp, col, val);
 }
 
 }  // namespace cr

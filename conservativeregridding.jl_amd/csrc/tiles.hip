@@ -109,8 +109,8 @@ struct TileAcc<1, float> {
 
 // UNI: segment ends are the plan's wave-uniform masks (kRounds plans); else
 // each lane finds them from its own slots (kEven plans).
-template <class T, int F, int MODE, bool OFF32, bool UNI>
-__global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
+template <class G, class T, int F, int MODE, bool OFF32, bool UNI>
+__global__ void __launch_bounds__(G::kThreads) k_spmm_tiles(
     const TileDesc *__restrict__ tiles, int64_t ntiles, int64_t tiles_per_xcd, int nchunks,
     int64_t chunk, int chunk_major, const uint32_t *__restrict__ keys,
     const typename TileAcc<MODE, T>::val *__restrict__ vals,
@@ -120,8 +120,8 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
     using Acc = typename TileAcc<MODE, T>::type;
     using V = typename TileAcc<MODE, T>::val;
     using Off = typename std::conditional<OFF32, uint32_t, int64_t>::type;
-    __shared__ Pack<T, F> xs[kTileCols];
-    __shared__ Pack<T, F> seg[kTileSegs + 1];
+    __shared__ Pack<T, F> xs[G::kCols];
+    __shared__ Pack<T, F> seg[G::kSegs + 1];
     const int64_t b = blockIdx.x;
     const int64_t local = b >> 3;
     // tile-major: the chunks of a tile run side by side; chunk-major: the whole
@@ -137,14 +137,14 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
     const int tid = threadIdx.x;
     const int E = d.ent;
 
-    uint32_t key[kTileEnt];
-    V v[kTileEnt];
+    uint32_t key[G::kEnt];
+    V v[G::kEnt];
 #pragma unroll
-    for (int e = 0; e < kTileEnt; ++e) {
+    for (int e = 0; e < G::kEnt; ++e) {
         key[e] = 0u;
         v[e] = V(0);
         if (e < E) {
-            const int64_t q = d.ent_off + (int64_t)e * kTileThreads + tid;
+            const int64_t q = d.ent_off + (int64_t)e * G::kThreads + tid;
             key[e] = keys[q];
             v[e] = vals[q];
         }
@@ -155,27 +155,27 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
         flush = __builtin_amdgcn_readfirstlane(d.flush[tid >> 6]);
     } else {
 #pragma unroll
-        for (int e = 0; e < kTileEnt; ++e) {
-            const uint32_t nxt = (e + 1 < kTileEnt) ? (key[e + 1] >> 16) : 0xffffffffu;
+        for (int e = 0; e < G::kEnt; ++e) {
+            const uint32_t nxt = (e + 1 < G::kEnt) ? (key[e + 1] >> 16) : 0xffffffffu;
             if (e < E && (e + 1 == E || nxt != (key[e] >> 16))) flush |= 1u << e;
         }
     }
-    Off xo[kColsPerThread];
-    bool xin[kColsPerThread];
+    Off xo[G::kColsPerThread];
+    bool xin[G::kColsPerThread];
 #pragma unroll
-    for (int j = 0; j < kColsPerThread; ++j) {
-        const int u = tid + j * kTileThreads;
+    for (int j = 0; j < G::kColsPerThread; ++j) {
+        const int u = tid + j * G::kThreads;
         xin[j] = u < d.ncols;
         xo[j] = xin[j] ? (Off)((int64_t)cols[d.col_off + u] * sc) : (Off)0;
     }
     const uint16_t *rs = rseg + d.row0 + tile;
-    int sb[kRowsPerThread], se[kRowsPerThread];
-    bool rin[kRowsPerThread];
-    Off yo[kRowsPerThread];
-    double rscale[kRowsPerThread];
+    int sb[G::kRowsPerThread], se[G::kRowsPerThread];
+    bool rin[G::kRowsPerThread];
+    Off yo[G::kRowsPerThread];
+    double rscale[G::kRowsPerThread];
 #pragma unroll
-    for (int i = 0; i < kRowsPerThread; ++i) {
-        const int r = tid + i * kTileThreads;
+    for (int i = 0; i < G::kRowsPerThread; ++i) {
+        const int r = tid + i * G::kThreads;
         rin[i] = r < d.nrows;
         sb[i] = 0;
         se[i] = 0;
@@ -188,22 +188,22 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
         }
     }
 
-    T pre[F][kColsPerThread];
+    T pre[F][G::kColsPerThread];
 #pragma unroll
     for (int f = 0; f < F; ++f) {
         const T *xb = x + (fb + f) * sf;  // uniform per field
         const bool fok = fb + f < fe;
 #pragma unroll
-        for (int j = 0; j < kColsPerThread; ++j) pre[f][j] = (xin[j] && fok) ? xb[xo[j]] : T(0);
+        for (int j = 0; j < G::kColsPerThread; ++j) pre[f][j] = (xin[j] && fok) ? xb[xo[j]] : T(0);
     }
 
     for (int64_t f0 = fb; f0 < fe; f0 += F) {
 #pragma unroll
-        for (int j = 0; j < kColsPerThread; ++j) {
+        for (int j = 0; j < G::kColsPerThread; ++j) {
             Pack<T, F> p;
 #pragma unroll
             for (int f = 0; f < F; ++f) p.v[f] = pre[f][j];
-            xs[tid + j * kTileThreads] = p;
+            xs[tid + j * G::kThreads] = p;
         }
         const int64_t fn = f0 + F;
         if (fn < fe) {
@@ -212,7 +212,7 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
                 const T *xb = x + (fn + f) * sf;
                 const bool fok = fn + f < fe;
 #pragma unroll
-                for (int j = 0; j < kColsPerThread; ++j) pre[f][j] = (xin[j] && fok) ? xb[xo[j]] : T(0);
+                for (int j = 0; j < G::kColsPerThread; ++j) pre[f][j] = (xin[j] && fok) ? xb[xo[j]] : T(0);
             }
         }
         __syncthreads();
@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
 #pragma unroll
             for (int f = 0; f < F; ++f) a[f].init();
 #pragma unroll
-            for (int e = 0; e < kTileEnt; ++e) {
+            for (int e = 0; e < G::kEnt; ++e) {
                 if (e < E) {
                     const Pack<T, F> xv = xs[key[e] & 0xffffu];
 #pragma unroll
@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
         __syncthreads();
         // rows: add their segments in order, store coalesced
 #pragma unroll
-        for (int i = 0; i < kRowsPerThread; ++i) {
+        for (int i = 0; i < G::kRowsPerThread; ++i) {
             if (rin[i]) {
                 double acc[F];
 #pragma unroll
@@ -256,7 +256,9 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
                 for (int f = 0; f < F; ++f) {
                     if (f0 + f < fe) {
                         T *yb = y + (f0 + f) * df;  // uniform per field
-                        yb[yo[i]] = row_finish(acc[f], rscale[i], scale != nullptr, (T *)nullptr);
+                        // written once: keep it from displacing source lines in L2
+                        __builtin_nontemporal_store(row_finish(acc[f], rscale[i], scale != nullptr, (T *)nullptr),
+                                                    yb + yo[i]);
                     }
                 }
             }
@@ -268,9 +270,9 @@ __global__ void __launch_bounds__(kTileThreads) k_spmm_tiles(
 }
 
 // ---- plan ----------------------------------------------------------------------------
-const TilePlan &tile_plan(const Csr &A, int style, hipStream_t s) {
+const TilePlan &tile_plan(const Csr &A, int style, int geom, hipStream_t s) {
     std::lock_guard<std::mutex> lock(A.plan_mu);
-    if (A.tiles[style]) return *A.tiles[style];
+    if (A.tiles[style][geom]) return *A.tiles[style][geom];
     auto P = std::make_shared<TilePlan>();
     const int64_t nrows = A.nrows, nnz = A.nnz;
     std::vector<int64_t> rp(nrows + 1);
@@ -282,7 +284,10 @@ const TilePlan &tile_plan(const Csr &A, int style, hipStream_t s) {
         CR_HIP(hipMemcpyAsync(val.data(), A.val.p, A.val.bytes(), hipMemcpyDeviceToHost, s));
     }
     CR_HIP(hipStreamSynchronize(s));
-    HostTilePlan H = make_host_tile_plan(style, nrows, A.ncols, rp.data(), col.data(), val.data());
+    HostTilePlan H = geom == kGeomL
+                         ? make_host_tile_plan<GeomL>(style, nrows, A.ncols, rp.data(), col.data(), val.data())
+                         : make_host_tile_plan<GeomS>(style, nrows, A.ncols, rp.data(), col.data(), val.data());
+    P->geom = geom;
 
     P->ntiles = (int64_t)H.desc.size();
     P->nslots = (int64_t)H.key.size();
@@ -305,18 +310,18 @@ const TilePlan &tile_plan(const Csr &A, int style, hipStream_t s) {
     up(P->rseg.p, H.rseg.data(), H.rseg.size() * sizeof(uint16_t));
     up(P->huge.p, H.huge.data(), H.huge.size() * sizeof(int32_t));
     CR_HIP(hipStreamSynchronize(s));
-    A.tiles[style] = P;
-    return *A.tiles[style];
+    A.tiles[style][geom] = P;
+    return *A.tiles[style][geom];
 }
 
 void drop_tile_plan(const Csr &A) {
     std::lock_guard<std::mutex> lock(A.plan_mu);
-    A.tiles[0].reset();
-    A.tiles[1].reset();
+    for (auto &byStyle : A.tiles)
+        for (auto &p : byStyle) p.reset();
 }
 
 // ---- launch --------------------------------------------------------------------------
-template <class T, int F, int MODE, bool UNI>
+template <class G, class T, int F, int MODE, bool UNI>
 static void launch_f(const TilePlan &P, int nchunks, int chunk_major, const T *x, int64_t sc, int64_t sf,
                      T *y, int64_t dc, int64_t df, int64_t nf, const double *scale, int64_t nrows,
                      int64_t ncols, hipStream_t s) {
@@ -332,12 +337,12 @@ static void launch_f(const TilePlan &P, int nchunks, int chunk_major, const T *x
     else vals = (const V *)P.val64.p;
     const bool off32 = (ncols - 1) * sc < ((int64_t)1 << 32) && (nrows - 1) * dc < ((int64_t)1 << 32);
     if (off32)
-        hipLaunchKernelGGL((k_spmm_tiles<T, F, MODE, true, UNI>), dim3((unsigned)blocks), dim3(kTileThreads), 0, s,
+        hipLaunchKernelGGL((k_spmm_tiles<G, T, F, MODE, true, UNI>), dim3((unsigned)blocks), dim3(G::kThreads), 0, s,
                            P.tiles.p, P.ntiles, tiles_per_xcd, nchunks, chunk, chunk_major, P.key.p, vals,
                            P.cols.p, P.rseg.p,
                            x, sc, sf, y, dc, df, nf, scale);
     else
-        hipLaunchKernelGGL((k_spmm_tiles<T, F, MODE, false, UNI>), dim3((unsigned)blocks), dim3(kTileThreads), 0, s,
+        hipLaunchKernelGGL((k_spmm_tiles<G, T, F, MODE, false, UNI>), dim3((unsigned)blocks), dim3(G::kThreads), 0, s,
                            P.tiles.p, P.ntiles, tiles_per_xcd, nchunks, chunk, chunk_major, P.key.p, vals,
                            P.cols.p, P.rseg.p,
                            x, sc, sf, y, dc, df, nf, scale);
@@ -346,7 +351,7 @@ static void launch_f(const TilePlan &P, int nchunks, int chunk_major, const T *x
 // Workgroups for a tile launch: enough (tile, chunk) pairs to keep every CU
 // busy several times over when the plan has few tiles.
 int default_chunks(const TilePlan &P, int64_t nf, int F) {
-    const int64_t want = 8 * 256;  // 8 workgroups per CU
+    const int64_t want = (P.geom == kGeomL ? 4 : 8) * 256;  // 32 waves per CU
     int64_t c = (want + P.ntiles - 1) / std::max<int64_t>(1, P.ntiles);
     c = std::max<int64_t>(1, std::min<int64_t>(c, (nf + F - 1) / F));
     return (int)c;
@@ -363,28 +368,35 @@ void launch_tiles(const TilePlan &P, int F, bool acc64, int nchunks, int chunk_m
     if (P.ntiles == 0) return;
     if (nchunks <= 0) nchunks = default_chunks(P, nf, F);
     nchunks = (int)std::max<int64_t>(1, std::min<int64_t>(nchunks, (nf + F - 1) / F));
+#define CR_TILE_ARGS P, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s
     if constexpr (sizeof(T) == 4) {
-        if (acc64) {
+        if (P.geom == kGeomL) {
+            CR_REQUIRE(acc64, CR_UNSUPPORTED, "1024-row tiles are built with FP64 sums only");
+            if (F == 2) launch_f<GeomL, T, 2, 1, true>(CR_TILE_ARGS);
+            else launch_f<GeomL, T, 4, 1, true>(CR_TILE_ARGS);
+        } else if (acc64) {
             switch (F) {
-                case 2: launch_f<T, 2, 1, true>(P, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
-                case 8: launch_f<T, 8, 1, true>(P, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
-                default: launch_f<T, 4, 1, true>(P, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
+                case 2: launch_f<GeomS, T, 2, 1, true>(CR_TILE_ARGS); break;
+                case 8: launch_f<GeomS, T, 8, 1, true>(CR_TILE_ARGS); break;
+                default: launch_f<GeomS, T, 4, 1, true>(CR_TILE_ARGS); break;
             }
         } else {
             switch (F) {
-                case 2: launch_f<T, 2, 0, true>(P, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
-                case 8: launch_f<T, 8, 0, true>(P, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
-                default: launch_f<T, 4, 0, true>(P, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
+                case 2: launch_f<GeomS, T, 2, 0, true>(CR_TILE_ARGS); break;
+                case 8: launch_f<GeomS, T, 8, 0, true>(CR_TILE_ARGS); break;
+                default: launch_f<GeomS, T, 4, 0, true>(CR_TILE_ARGS); break;
             }
         }
     } else {
         (void)acc64;
+        CR_REQUIRE(P.geom == kGeomS, CR_UNSUPPORTED, "FP64 fields use 512-row tiles");
         switch (F) {
-            case 1: launch_f<T, 1, 0, false>(P, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
-            case 2: launch_f<T, 2, 0, false>(P, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
-            default: launch_f<T, 4, 0, false>(P, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s); break;
+            case 1: launch_f<GeomS, T, 1, 0, false>(CR_TILE_ARGS); break;
+            case 2: launch_f<GeomS, T, 2, 0, false>(CR_TILE_ARGS); break;
+            default: launch_f<GeomS, T, 4, 0, false>(CR_TILE_ARGS); break;
         }
     }
+#undef CR_TILE_ARGS
 }
 
 template void launch_tiles<float>(const TilePlan &, int, bool, int, int, const float *, int64_t, int64_t,

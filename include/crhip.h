@@ -210,8 +210,10 @@ cr_status cr_regrid_tune(cr_regridder *R, int32_t transpose, int64_t nfields, co
  *           tile kernel with 2, 4, 8 fields per LDS stage and FP32 Kahan sums;
  *           8-10 the tile kernel with 2, 4, 8 fields per stage and FP64 sums
  *           (default 9); 11-14 variant 9 with 4 field chunks, 8 and 16 chunks
- *           swept one at a time, and a single chunk.  FP32 fields with unit
- *           field strides and >= 64 fields always use the fields-fastest kernel.
+ *           swept one at a time, and a single chunk; 15-17 FP64 sums on
+ *           1024-row tiles (512 threads) with 4, 2 and 4 fields per stage (17:
+ *           4 field chunks).  FP32 fields with unit field strides and >= 64
+ *           fields always use the fields-fastest kernel.
  *   CR_F64: 0 row kernel, 1 staged tile kernel (default).
  */
 int32_t cr_regrid_variant_count(int32_t dtype);
