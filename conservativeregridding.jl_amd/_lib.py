@@ -88,6 +88,15 @@ SIGNATURES = {
 
 
 def _load():
+    # PyTorch bundles its own HIP runtime under the same soname
+    # (libamdhip64.so.7) as the one libcrhip.so links; whichever is loaded
+    # first serves the whole process.  Load torch's first, so that torch and
+    # this library share one runtime and torch keeps the runtime it was built
+    # with (loading ours first leaves torch without devices).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is not built: the HIP library is required (no CPU fallback). "

@@ -66,30 +66,274 @@ __global__ void k_build_level(const float4 *__restrict__ child, int64_t nchild,
 }
 
 // Candidate search, one wave per 64 destination cells taken in space-filling-
-// curve order of their caps (a compact cluster).  The wave bounds its 64 caps by one cap,
-// walks the source tree level by level with that cap (lanes test the children
-// of the current frontier side by side; ballot + prefix count compact the
-// survivors into LDS), and ends with the list of source leaves that touch the
-// cluster.  Each lane then tests its own cap against that list (LDS broadcast
-// reads, no divergence).  A cluster whose frontier or leaf list does not fit
-// the LDS lists falls back to a per-lane stackless walk (tree_query).  Every
-// leaf that touches a cell's cap touches the cluster cap (it contains the
-// cell's cap) and all of its ancestors, so both routes emit exactly the
+// curve order of their caps (a compact cluster).  The wave bounds its 64 caps
+// by one cap, walks the source tree level by level with that cap (lanes test
+// the children of the current frontier side by side; ballot + prefix count
+// compact the survivors into LDS), and ends with the list of source leaves
+// that touch the cluster.  Each lane then tests its own cap against that list
+// (LDS broadcast reads, no divergence).  A cluster whose frontier or leaf list
+// overflows the LDS lists is retried as 4 clusters of 16 lanes; the cells of
+// a 16-lane cluster that still overflows are walked one at a time by the
+// whole wave, which counts or writes the hits straight from its ballots (no
+// leaf list); only a cell whose frontier overflows even then walks the tree
+// alone (tree_query).  Every leaf that touches a cell's cap touches any cluster cap
+// that contains it and all of its ancestors, so every route emits exactly the
 // leaves whose caps touch the cell's cap, in a fixed order.  FILL=false
 // counts; FILL=true writes at the scanned offsets (never past the cell's
 // segment).
 constexpr int kSearchLeaves = 1024;
 constexpr int kSearchFront = 256;
 
-__device__ __forceinline__ float wave_sum(float v) {
+template <int W>
+__device__ __forceinline__ float group_sum(float v) {
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    for (int m = W / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
     return v;
 }
-__device__ __forceinline__ float wave_max(float v) {
+template <int W>
+__device__ __forceinline__ float group_max(float v) {
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, 64));
+    for (int m = W / 2; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, 64));
     return v;
+}
+
+// Cap around the caps of the W-lane group of this lane (lanes whose `in` is
+// false are left out); w < 0 marks an empty group, w >= 2 a group too wide to
+// bound (both handled as overflow).  Uniform over each group.
+template <int W>
+__device__ float4 group_cap(const float4 &c, bool in) {
+    const float sx = group_sum<W>(in ? c.x : 0.f), sy = group_sum<W>(in ? c.y : 0.f),
+                sz = group_sum<W>(in ? c.z : 0.f);
+    const float L = sqrtf(sx * sx + sy * sy + sz * sz);
+    const float wide = group_max<W>(in && c.w >= 2.0f ? 1.f : 0.f);
+    const float cnt = group_sum<W>(in ? 1.f : 0.f);
+    if (cnt == 0.f) return make_float4(0.f, 0.f, 1.f, -1.f);
+    if (wide > 0.f || L < 1e-3f) return make_float4(0.f, 0.f, 1.f, kCapAll);
+    const float mx = sx / L, my = sy / L, mz = sz / L;
+    float dd = 0.f;
+    if (in) {
+        const float dx = c.x - mx, dy = c.y - my, dz = c.z - mz;
+        dd = sqrtf(dx * dx + dy * dy + dz * dz) + c.w;
+    }
+    return make_float4(mx, my, mz, group_max<W>(dd) * (1.0f + 1e-5f) + 1e-5f);
+}
+
+__device__ __forceinline__ float4 shfl4(const float4 &v, int src) {
+    return make_float4(__shfl(v.x, src, 64), __shfl(v.y, src, 64), __shfl(v.z, src, 64), __shfl(v.w, src, 64));
+}
+
+// The whole wave walks the tree with the (wave-uniform) cap wc and lists the
+// leaves that touch it in lcap/lid.  Returns the count, or -1 when a frontier
+// or the leaf list overflows (or wc cannot be bounded).
+__device__ int coop_collect(const float4 &wc, const float4 *__restrict__ nodes, const Levels &lv,
+                            const int32_t *__restrict__ perm, float4 *lcap, int32_t *lid,
+                            int32_t (*front)[kSearchFront]) {
+    const int lane = threadIdx.x;
+    if (wc.w < 0.0f || wc.w >= 1.4f) return -1;
+    if (lv.top == 0) {  // a single leaf
+        __syncthreads();
+        if (lane == 0) {
+            lcap[0] = nodes[0];
+            lid[0] = perm[0];
+        }
+        __syncthreads();
+        return 1;
+    }
+    __syncthreads();  // the previous list is no longer read
+    if (lane == 0) front[0][0] = 0;
+    __syncthreads();
+    int cur = 0, nf = 1;
+    for (int lev = lv.top; lev >= 1; --lev) {
+        const int64_t csize = lv.size[lev - 1], coff = lv.off[lev - 1];
+        const int tot = nf * kTreeFanout;
+        int nn = 0;
+        // four chunks of 64 children per step: their loads are issued together
+        for (int base = 0; base < tot; base += 4 * 64) {
+            int64_t child[4];
+            float4 cc[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int t = base + 64 * u + lane;
+                child[u] = -1;
+                cc[u] = make_float4(0.f, 0.f, 0.f, -8.0f);
+                if (t < tot) {
+                    const int64_t ch = (int64_t)front[cur][t >> 3] * kTreeFanout + (t & 7);
+                    if (ch < csize) {
+                        child[u] = ch;
+                        cc[u] = nodes[coff + ch];
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const bool hit = child[u] >= 0 && caps_overlap(wc, cc[u]);
+                const uint64_t m = __ballot(hit);
+                const int pos = nn + (int)__popcll(m & ((1ull << lane) - 1ull));
+                if (hit) {
+                    if (lev > 1) {
+                        if (pos < kSearchFront) front[cur ^ 1][pos] = (int32_t)child[u];
+                    } else if (pos < kSearchLeaves) {
+                        lcap[pos] = cc[u];
+                        lid[pos] = perm[child[u]];
+                    }
+                }
+                nn += (int)__popcll(m);
+            }
+        }
+        __syncthreads();
+        if (nn > (lev > 1 ? kSearchFront : kSearchLeaves)) return -1;
+        cur ^= 1;
+        nf = nn;
+    }
+    return nf;
+}
+
+// One lane's tests against the collected list (four independent tests per
+// step: a wave per SIMD has no other wave to hide a dependent chain).
+template <bool FILL>
+__device__ void lane_emit(const float4 &c, int64_t d, int nleaf, const float4 *lcap, const int32_t *lid,
+                          int64_t *__restrict__ count_or_offset, int32_t *__restrict__ cand_src,
+                          int32_t *__restrict__ cand_dst) {
+    if (FILL) {
+        int64_t pos = count_or_offset[d];
+        const int64_t end = count_or_offset[d + 1];
+        int j = 0;
+        for (; j + 4 <= nleaf; j += 4) {
+            bool h[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) h[u] = caps_overlap(c, lcap[j + u]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (h[u] && pos < end) {
+                    cand_src[pos] = lid[j + u];
+                    cand_dst[pos] = (int32_t)d;
+                    ++pos;
+                }
+        }
+        for (; j < nleaf; ++j)
+            if (caps_overlap(c, lcap[j]) && pos < end) {
+                cand_src[pos] = lid[j];
+                cand_dst[pos] = (int32_t)d;
+                ++pos;
+            }
+    } else {
+        int c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+        int j = 0;
+        for (; j + 4 <= nleaf; j += 4) {
+            c0 += caps_overlap(c, lcap[j]);
+            c1 += caps_overlap(c, lcap[j + 1]);
+            c2 += caps_overlap(c, lcap[j + 2]);
+            c3 += caps_overlap(c, lcap[j + 3]);
+        }
+        for (; j < nleaf; ++j) c0 += caps_overlap(c, lcap[j]);
+        count_or_offset[d] = (int64_t)(c0 + c1 + c2 + c3);
+    }
+}
+
+template <bool FILL>
+__device__ void lane_walk(const float4 &c, int64_t d, const float4 *__restrict__ nodes, const Levels &lv,
+                          const int32_t *__restrict__ perm, int64_t *__restrict__ count_or_offset,
+                          int32_t *__restrict__ cand_src, int32_t *__restrict__ cand_dst) {
+    if (FILL) {
+        int64_t pos = count_or_offset[d];
+        const int64_t end = count_or_offset[d + 1];
+        tree_query(c, nodes, lv, [&](int64_t leaf) {
+            if (pos < end) {
+                cand_src[pos] = perm[leaf];
+                cand_dst[pos] = (int32_t)d;
+                ++pos;
+            }
+        });
+    } else {
+        int64_t cnt = 0;
+        tree_query(c, nodes, lv, [&](int64_t) { ++cnt; });
+        count_or_offset[d] = cnt;
+    }
+}
+
+// One cell (cap qc, row d, wave-uniform) walked by the whole wave; the leaf
+// level is tested 64 children at a time and the hits are counted, or written
+// in order at the cell's offset, straight from the ballot (no leaf list).
+// The frontier lives in `scratch` (kSearchBigFront entries).  Returns false
+// when a frontier overflows (nothing was counted or written then).
+constexpr int kSearchBigFront = kSearchLeaves * 4;  // int32 entries in the lcap storage
+
+template <bool FILL>
+__device__ bool coop_emit_one(const float4 &qc, int64_t d, const float4 *__restrict__ nodes, const Levels &lv,
+                              const int32_t *__restrict__ perm, int32_t *scratch,
+                              int64_t *__restrict__ count_or_offset, int32_t *__restrict__ cand_src,
+                              int32_t *__restrict__ cand_dst) {
+    const int lane = threadIdx.x;
+    int32_t *fa = scratch, *fb = scratch + kSearchBigFront / 2;
+    const int cap = kSearchBigFront / 2;
+    __syncthreads();
+    if (lane == 0) fa[0] = 0;
+    __syncthreads();
+    int nf = 1;
+    int64_t pos = 0, end = 0;
+    if (FILL) {
+        pos = count_or_offset[d];
+        end = count_or_offset[d + 1];
+    }
+    int64_t cnt = 0;
+    if (lv.top == 0) {
+        const bool hit = lane == 0 && caps_overlap(qc, nodes[0]);
+        if (hit) {
+            if (FILL && pos < end) {
+                cand_src[pos] = perm[0];
+                cand_dst[pos] = (int32_t)d;
+            }
+        }
+        cnt = (int64_t)__popcll(__ballot(hit));
+    }
+    for (int lev = lv.top; lev >= 1; --lev) {
+        const int64_t csize = lv.size[lev - 1], coff = lv.off[lev - 1];
+        const int tot = nf * kTreeFanout;
+        int nn = 0;
+        for (int base = 0; base < tot; base += 4 * 64) {
+            int64_t child[4];
+            float4 cc[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int t = base + 64 * u + lane;
+                child[u] = -1;
+                cc[u] = make_float4(0.f, 0.f, 0.f, -8.0f);
+                if (t < tot) {
+                    const int64_t ch = (int64_t)fa[t >> 3] * kTreeFanout + (t & 7);
+                    if (ch < csize) {
+                        child[u] = ch;
+                        cc[u] = nodes[coff + ch];
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const bool hit = child[u] >= 0 && caps_overlap(qc, cc[u]);
+                const uint64_t m = __ballot(hit);
+                const int rank = (int)__popcll(m & ((1ull << lane) - 1ull));
+                if (lev > 1) {
+                    if (hit && nn + rank < cap) fb[nn + rank] = (int32_t)child[u];
+                    nn += (int)__popcll(m);
+                } else {
+                    if (FILL && hit && pos + cnt + rank < end) {
+                        cand_src[pos + cnt + rank] = perm[child[u]];
+                        cand_dst[pos + cnt + rank] = (int32_t)d;
+                    }
+                    cnt += (int64_t)__popcll(m);
+                }
+            }
+        }
+        __syncthreads();
+        if (lev > 1) {
+            if (nn > cap) return false;
+            int32_t *t = fa;
+            fa = fb;
+            fb = t;
+            nf = nn;
+        }
+    }
+    if (!FILL && lane == 0) count_or_offset[d] = cnt;
+    return true;
 }
 
 template <bool FILL>
@@ -109,108 +353,40 @@ __global__ void __launch_bounds__(64) k_search(const float4 *__restrict__ qcap,
     const int64_t d = active ? qorder[i] : 0;
     const float4 c = active ? qcap[d] : make_float4(0.f, 0.f, 0.f, 0.f);
 
-    // cluster cap: centre = normalised sum of centres, radius = max(|c - m| + r)
-    const float sx = wave_sum(active ? c.x : 0.f), sy = wave_sum(active ? c.y : 0.f),
-                sz = wave_sum(active ? c.z : 0.f);
-    const float L = sqrtf(sx * sx + sy * sy + sz * sz);
-    const bool wide = __ballot(active && c.w >= 2.0f) != 0ull;
-    bool overflow = wide || L < 1e-3f;
-    int nleaf = 0;
-    if (!overflow) {
-        const float mx = sx / L, my = sy / L, mz = sz / L;
-        float dd = 0.f;
-        if (active) {
-            const float dx = c.x - mx, dy = c.y - my, dz = c.z - mz;
-            dd = sqrtf(dx * dx + dy * dy + dz * dz) + c.w;
-        }
-        const float rad = wave_max(dd) * (1.0f + 1e-5f) + 1e-5f;
-        overflow = rad >= 1.4f;
-        const float4 wc = make_float4(mx, my, mz, rad);
-        if (!overflow) {
-            if (lv.top == 0) {  // a single leaf
-                if (lane == 0) {
-                    lcap[0] = nodes[0];
-                    lid[0] = perm[0];
-                }
-                nleaf = 1;
-            } else {
-                if (lane == 0) front[0][0] = 0;
-                __syncthreads();
-                int cur = 0, nf = 1;
-                for (int lev = lv.top; lev >= 1; --lev) {
-                    const int64_t csize = lv.size[lev - 1], coff = lv.off[lev - 1];
-                    const int tot = nf * kTreeFanout;
-                    int nn = 0;
-                    for (int base = 0; base < tot; base += 64) {
-                        const int t = base + lane;
-                        bool hit = false;
-                        int64_t child = 0;
-                        float4 cc = make_float4(0.f, 0.f, 0.f, 0.f);
-                        if (t < tot) {
-                            child = (int64_t)front[cur][t >> 3] * kTreeFanout + (t & 7);
-                            if (child < csize) {
-                                cc = nodes[coff + child];
-                                hit = caps_overlap(wc, cc);
-                            }
-                        }
-                        const uint64_t m = __ballot(hit);
-                        const int pos = nn + (int)__popcll(m & ((1ull << lane) - 1ull));
-                        if (hit) {
-                            if (lev > 1) {
-                                if (pos < kSearchFront) front[cur ^ 1][pos] = (int32_t)child;
-                            } else if (pos < kSearchLeaves) {
-                                lcap[pos] = cc;
-                                lid[pos] = perm[child];
-                            }
-                        }
-                        nn += (int)__popcll(m);
-                    }
-                    __syncthreads();
-                    if (nn > (lev > 1 ? kSearchFront : kSearchLeaves)) {
-                        overflow = true;
-                        break;
-                    }
-                    cur ^= 1;
-                    nf = nn;
-                }
-                nleaf = nf;
-            }
-        }
-    }
-    __syncthreads();
-    if (!active) return;
-    if (overflow) {
-        if (FILL) {
-            int64_t pos = count_or_offset[d];
-            const int64_t end = count_or_offset[d + 1];
-            tree_query(c, nodes, lv, [&](int64_t leaf) {
-                if (pos < end) {
-                    cand_src[pos] = perm[leaf];
-                    cand_dst[pos] = (int32_t)d;
-                    ++pos;
-                }
-            });
-        } else {
-            int64_t cnt = 0;
-            tree_query(c, nodes, lv, [&](int64_t) { ++cnt; });
-            count_or_offset[d] = cnt;
-        }
+    // the 64-cell cluster
+    const float4 wc = group_cap<64>(c, active);
+    int n = coop_collect(wc, nodes, lv, perm, lcap, lid, front);
+    if (n >= 0) {
+        if (active) lane_emit<FILL>(c, d, n, lcap, lid, count_or_offset, cand_src, cand_dst);
         return;
     }
-    if (FILL) {
-        int64_t pos = count_or_offset[d];
-        const int64_t end = count_or_offset[d + 1];
-        for (int j = 0; j < nleaf; ++j)
-            if (caps_overlap(c, lcap[j]) && pos < end) {
-                cand_src[pos] = lid[j];
-                cand_dst[pos] = (int32_t)d;
-                ++pos;
-            }
-    } else {
-        int64_t cnt = 0;
-        for (int j = 0; j < nleaf; ++j) cnt += caps_overlap(c, lcap[j]);
-        count_or_offset[d] = cnt;
+    // four 16-cell clusters, one after the other; the lanes of a cluster that
+    // still overflows walk the tree alone, all together at the end
+    const float4 gcap = group_cap<16>(c, active);
+    bool walk = false;
+    for (int g = 0; g < 4; ++g) {
+        const float4 gc = shfl4(gcap, 16 * g);
+        n = coop_collect(gc, nodes, lv, perm, lcap, lid, front);
+        if ((lane >> 4) != g) continue;
+        if (n >= 0) {
+            if (active) lane_emit<FILL>(c, d, n, lcap, lid, count_or_offset, cand_src, cand_dst);
+        } else {
+            walk = true;
+        }
     }
+    // cells left over: one at a time, each walked by the whole wave
+    uint64_t left = __ballot(walk && active);
+    bool alone = false;
+    while (left) {
+        const int L = (int)__ffsll((long long)left) - 1;
+        left &= left - 1;
+        const float4 qc = shfl4(c, L);
+        const int64_t dq = (int64_t)__shfl((int)d, L, 64);
+        if (!coop_emit_one<FILL>(qc, dq, nodes, lv, perm, (int32_t *)lcap, count_or_offset, cand_src, cand_dst) &&
+            lane == L)
+            alone = true;
+    }
+    if (alone) lane_walk<FILL>(c, d, nodes, lv, perm, count_or_offset, cand_src, cand_dst);
 }
 
 template <int NC>
