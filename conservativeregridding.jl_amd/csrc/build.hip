@@ -11,7 +11,9 @@
 //        cell against the cluster's leaf list; count pass + scan + fill pass
 //        emit candidates grouped by destination row in a deterministic order
 //   compute_intersection_areas with ConvexConvexSutherlandHodgman + GO.area
-//     -> k_clip<4>: one thread per candidate, streaming FP64 clip + area
+//     -> k_clip_compact<4>: a uniform outside-one-edge test settles about a
+//        quarter of the candidates; the rest are queued in LDS and clipped 64
+//        at a time, one thread per candidate: streaming FP64 clip + area
 //   keep iff area > 0 (here > threshold_rel * min(a_src, a_dst)) and
 //   sparse(dst, src, areas)
 //     -> k_keep_count / k_keep_scatter: wave64 ballot compaction of kept pairs;
@@ -389,6 +391,79 @@ __global__ void __launch_bounds__(64) k_search(const float4 *__restrict__ qcap,
     if (alone) lane_walk<FILL>(c, d, nodes, lv, perm, count_or_offset, cand_src, cand_dst);
 }
 
+// Candidates whose subject lies outside one clip edge (about a quarter on the
+// benchmark grids) are settled by a cheap uniform test; the others are
+// queued in LDS and clipped 64 at a time, so every lane of the wave runs the
+// expensive pipeline on a real pair.  One wave per kClipChunks * 64
+// candidates; results go to each candidate's own slot.  (Used with a positive
+// drop threshold: a pair the test settles could at most leave a sub-rounding
+// sliver in the pipeline, which the threshold drops too.)
+constexpr int kClipChunks = 8;
+
+template <int NC>
+__device__ __forceinline__ void clip_candidate(int64_t c, const int32_t *__restrict__ cand_dst,
+                                               const int32_t *__restrict__ cand_src,
+                                               const double *__restrict__ dst_cells, int dst_nv,
+                                               const double *__restrict__ src_cells, int src_nv,
+                                               int clip_is_dst, const double *__restrict__ a_dst,
+                                               const double *__restrict__ a_src, double thr,
+                                               double *__restrict__ val) {
+    const int64_t d = cand_dst[c], s = cand_src[c];
+    const double *dp = dst_cells + 3 * (int64_t)dst_nv * d;
+    const double *sp = src_cells + 3 * (int64_t)src_nv * s;
+    const double *clip = clip_is_dst ? dp : sp, *subj = clip_is_dst ? sp : dp;
+    const int nc = clip_is_dst ? dst_nv : src_nv, ns = clip_is_dst ? src_nv : dst_nv;
+    const double a = clip_area_stream<NC>(clip, nc, subj, ns);
+    const double amin = fmin(a_src[s], a_dst[d]);
+    val[c] = a > thr * amin ? a : 0.0;
+}
+
+template <int NC>
+__global__ void __launch_bounds__(64) k_clip_compact(const int32_t *__restrict__ cand_dst,
+                                                     const int32_t *__restrict__ cand_src, int64_t ncand,
+                                                     const double *__restrict__ dst_cells, int dst_nv,
+                                                     const double *__restrict__ src_cells, int src_nv,
+                                                     int clip_is_dst, const double *__restrict__ a_dst,
+                                                     const double *__restrict__ a_src, double thr,
+                                                     double *__restrict__ val) {
+    __shared__ int32_t queue[128];
+    const int lane = threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * (64 * kClipChunks);
+    int qn = 0;
+    for (int ch = 0; ch <= kClipChunks; ++ch) {
+        if (ch < kClipChunks) {
+            const int64_t c = base + ch * 64 + lane;
+            bool keep = false;
+            if (c < ncand) {
+                const int64_t d = cand_dst[c], s = cand_src[c];
+                const double *dp = dst_cells + 3 * (int64_t)dst_nv * d;
+                const double *sp = src_cells + 3 * (int64_t)src_nv * s;
+                const double *clip = clip_is_dst ? dp : sp, *subj = clip_is_dst ? sp : dp;
+                const int nc = clip_is_dst ? dst_nv : src_nv, ns = clip_is_dst ? src_nv : dst_nv;
+                keep = !outside_one_edge<NC>(clip, nc, subj, ns);
+                if (!keep) val[c] = 0.0;
+            }
+            const uint64_t m = __ballot(keep);
+            if (keep) queue[qn + (int)__popcll(m & ((1ull << lane) - 1ull))] = ch * 64 + lane;
+            qn += (int)__popcll(m);
+            __syncthreads();
+        }
+        // a full batch, or what is left after the last chunk (one call site:
+        // the pipeline's code exists once)
+        if (qn >= 64 || (ch == kClipChunks && qn > 0)) {
+            if (lane < qn)
+                clip_candidate<NC>(base + queue[lane], cand_dst, cand_src, dst_cells, dst_nv, src_cells, src_nv,
+                                   clip_is_dst, a_dst, a_src, thr, val);
+            const int left = qn > 64 ? qn - 64 : 0;
+            const int moved = lane < left ? queue[64 + lane] : 0;
+            __syncthreads();
+            if (lane < left) queue[lane] = moved;
+            qn = left;
+            __syncthreads();
+        }
+    }
+}
+
 template <int NC>
 __global__ void __launch_bounds__(256) k_clip(const int32_t *__restrict__ cand_dst,
                                               const int32_t *__restrict__ cand_src, int64_t ncand,
@@ -714,7 +789,11 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     // 4. clip + area per candidate
     DevBuf<double> cval(ncand, s);
     if (ncand) {
-        if (clip_nv <= 4)
+        if (clip_nv <= 4 && o.threshold_rel > 0.0)
+            hipLaunchKernelGGL(k_clip_compact<4>, dim3(grid_for(ncand, 64 * kClipChunks)), dim3(64), 0, s,
+                               cand_dst.p, cand_src.p, ncand, dst->cells.p, dst->nv, src->cells.p, src->nv,
+                               clip_is_dst, a_dst.p, a_src.p, o.threshold_rel, cval.p);
+        else if (clip_nv <= 4)
             hipLaunchKernelGGL(k_clip<4>, dim3(grid_for(ncand, B)), dim3(B), 0, s, cand_dst.p,
                                cand_src.p, ncand, dst->cells.p, dst->nv, src->cells.p, src->nv,
                                clip_is_dst, a_dst.p, a_src.p, o.threshold_rel, cval.p);

@@ -100,9 +100,20 @@ struct ClipPipe {
     CR_HD void init() { acc.init(); }
 };
 
+// 1/sqrt(v): the device's rsqrt (one instruction sequence, no IEEE division);
+// the host build divides.  Either is within an ulp or two, which only moves a
+// renormalised point along its ray.
+CR_HD double inv_norm(double v) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return rsqrt(v);
+#else
+    return 1.0 / sqrt(v);
+#endif
+}
+
 CR_HD d3 crossing(const d3 &p, double sp, const d3 &q, double sq) {
     d3 x = fabs(sq) * p + fabs(sp) * q;
-    return (1.0 / sqrt(dot(x, x))) * x;
+    return inv_norm(dot(x, x)) * x;
 }
 
 template <int K, int NC>
@@ -176,6 +187,32 @@ CR_HD double clip_area_stream(const double *clip, int nc, const double *subj, in
     }
     pipe_close<0, NC>(P);
     return P.acc.area();
+}
+
+// True when every subject vertex lies strictly outside the great circle of
+// one clip edge (the pipeline's own side test, n.(v - a) < 0; a degenerate
+// edge rejects nothing): the intersection is then empty.
+template <int NC>
+CR_HD bool outside_one_edge(const double *clip, int nc, const double *subj, int ns) {
+    d3 a[NC], n[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) a[k] = load3(clip + 3 * (k < nc ? k : nc - 1));
+    bool out[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        d3 nk = cross(a[k], a[(k + 1) % NC] - a[k]);
+        out[k] = dot(nk, nk) >= kDegenerateEdge2;
+        n[k] = nk;
+    }
+    for (int i = 0; i < ns; ++i) {
+        const d3 v = load3(subj + 3 * i);
+#pragma unroll
+        for (int k = 0; k < NC; ++k) out[k] = out[k] && dot(n[k], v - a[k]) < 0.0;
+    }
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) any = any || out[k];
+    return any;
 }
 
 // Signed area of an open ring of n vertices.
