@@ -86,13 +86,16 @@ def apply_bytes(nnz, n_rows, n_cols, k, elem):
     return nnz * 12 + (n_rows + 1) * 8 + n_rows * 8 + k * (n_cols + n_rows) * elem
 
 
-def read_traffic(name):
-    """HBM bytes per launch from a committed rocprofv3 PMC summary, if present."""
+def read_traffic(name, variant):
+    """HBM bytes per launch of the given kernel variant from a committed
+    rocprofv3 PMC summary (lines ``traffic_bytes_per_launch[<variant>]: <bytes>``),
+    or None when the summary has no figure for that variant."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
+    key = f"traffic_bytes_per_launch[{variant}]:"
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key):
             return float(line.split(":", 1)[1])
     return None
 
@@ -335,7 +338,7 @@ def main():
                      "fp64_conservation_drift": drift64},
         "roofline": {"bound": "hbm", "achieved": round(gbps, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(gbps / HBM_PEAK_GBPS, 4),
-                     "traffic": read_traffic("spmm_f32_cells_c3_pmc.txt")},
+                     "traffic": read_traffic("spmm_f32_cells_c3_pmc.txt", variant)},
         "cpu_baseline": cpu,
         "device": dev,
     }
