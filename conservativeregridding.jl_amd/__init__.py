@@ -8,13 +8,14 @@ from ._lib import CRError, LIB_PATH, lib as _lib_handle
 from .errors import ArgumentError, DimensionMismatch, MethodError
 from .grids import (Grid, NESTED, RING, as_grid, cubed_sphere_points, healpix_cells,
                     healpix_nest2ring, healpix_ring2nest, lonlat_points, points_to_cells)
+from .esmf import save_esmf_weights, write_esmf_arrays
 from .regridder import (EARTH_RADIUS, Regridder, areas, normalize_, regrid, regrid_, regrid_plan,
                         regrid_tune, regrid_backward, regrid_variants, regrid_set_variant,
                         size, transpose)
 
 __all__ = [
     "Regridder", "regrid_", "regrid", "regrid_plan", "regrid_tune", "regrid_backward", "regrid_variants",
-    "regrid_set_variant", "transpose", "normalize_", "areas", "size",
+    "regrid_set_variant", "transpose", "normalize_", "areas", "size", "save_esmf_weights", "write_esmf_arrays",
     "Grid", "NESTED", "RING", "as_grid", "lonlat_points", "cubed_sphere_points", "healpix_cells",
     "healpix_nest2ring", "healpix_ring2nest", "points_to_cells",
     "ArgumentError", "DimensionMismatch", "MethodError", "CRError", "EARTH_RADIUS", "LIB_PATH",

@@ -47,6 +47,12 @@ class BuildStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class EsmfAttrs(ctypes.Structure):
+    _fields_ = [("src_grid_name", ctypes.c_char_p), ("dst_grid_name", ctypes.c_char_p),
+                ("created_at", ctypes.c_char_p), ("src_shape", ctypes.POINTER(c_i64)), ("src_ndim", c_i32),
+                ("dst_ndim", c_i32), ("dst_shape", ctypes.POINTER(c_i64))]
+
+
 # name -> (restype, argtypes); exactly the entry points of include/crhip.h
 SIGNATURES = {
     "cr_last_error": (ctypes.c_char_p, []),
@@ -83,6 +89,9 @@ SIGNATURES = {
     "cr_regrid_tune": (c_i32, [c_vp, c_i32, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, _i32p,
                                ctypes.POINTER(ctypes.c_float), c_vp]),
     "cr_regrid_variant_count": (c_i32, [c_i32]),
+    "cr_regridder_write_esmf": (c_i32, [c_vp, c_i32, ctypes.c_char_p, ctypes.POINTER(EsmfAttrs)]),
+    "cr_esmf_write": (c_i32, [ctypes.c_char_p, c_i64, _dp, _i32p, _i32p, c_i64, _dp, _dp, c_i64, _dp, _dp,
+                              ctypes.POINTER(EsmfAttrs)]),
     "cr_regrid_set_variant": (c_i32, [c_vp, c_i32, c_i32, c_i32]),
 }
 

@@ -219,6 +219,38 @@ cr_status cr_regrid_tune(cr_regridder *R, int32_t transpose, int64_t nfields, co
 int32_t cr_regrid_variant_count(int32_t dtype);
 cr_status cr_regrid_set_variant(cr_regridder *R, int32_t transpose, int32_t dtype, int32_t variant);
 
+/* ------------------------------------------------- ESMF weights files -- */
+/*
+ * save_esmf_weights(path, regridder; src_grid_name, dst_grid_name, src_shape,
+ * dst_shape, created_at) of ext/ConservativeRegriddingNCDatasetsExt.jl:15-59:
+ * an ESMF offline-weights file with dimensions n_s, n_a, n_b, variables S =
+ * A ./ dst_areas[row], row, col (1-based, column-major order of A), frac_a,
+ * frac_b, area_a, area_b and the same global attributes.  Written as netCDF
+ * classic with 64-bit offsets (CDF-2); row, col and the grid shapes are 32-bit
+ * integers (the classic format has no 64-bit integer type).  Strings and
+ * shapes in cr_esmf_attrs may be NULL / 0 (defaults "source", "destination";
+ * created_at and the shapes are then omitted, as in the reference).
+ */
+typedef struct {
+    const char *src_grid_name;
+    const char *dst_grid_name;
+    const char *created_at;
+    const int64_t *src_shape;
+    int32_t src_ndim;
+    int32_t dst_ndim;
+    const int64_t *dst_shape;
+} cr_esmf_attrs;
+/* The file of this direction of R (transpose=1: transpose(R)); host work only
+ * (synchronous copies of A and the areas). */
+cr_status cr_regridder_write_esmf(const cr_regridder *R, int32_t transpose, const char *path,
+                                  const cr_esmf_attrs *attrs);
+/* The same file from host arrays (S, row, col of length n_s; frac_a, area_a of
+ * n_a; frac_b, area_b of n_b); needs no GPU. */
+cr_status cr_esmf_write(const char *path, int64_t n_s, const double *S, const int32_t *row,
+                        const int32_t *col, int64_t n_a, const double *frac_a, const double *area_a,
+                        int64_t n_b, const double *frac_b, const double *area_b,
+                        const cr_esmf_attrs *attrs);
+
 #ifdef __cplusplus
 }
 #endif
