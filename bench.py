@@ -243,6 +243,20 @@ def main():
     gbps_loop = K * B64 / (ms_loop * 1e-3) / 1e9
     log(f"[bench] fastest FP64 loop ({best64}) {ms_loop:.3f} ms; batched speed-up {ms_loop / ms_step:.2f}x")
 
+    # ---- reverse mode: field adjoint + vertex gradients of one FP64 field -------
+    x0, y0 = X64[0].contiguous(), Y64[0].contiguous()
+    gy0 = torch.ones_like(y0)
+    cr.regrid_backward(R, dst, src, x0, y0, gy0)  # warm-up (plans of the transpose direction)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(3):
+        cr.regrid_backward(R, dst, src, x0, y0, gy0)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_bwd = e0.elapsed_time(e1) / 3
+    log(f"[bench] backward (field adjoint + vertex gradients, one FP64 field): {ms_bwd:.3f} ms "
+        f"({nnz / (ms_bwd * 1e-3) / 1e6:.0f} M pairs/s)")
+
     # ---- plain copy of a known size (HBM reference and counter calibration) ----
     cpy_n = 1 << 27  # 128 Mi floats = 512 MiB read + 512 MiB written
     ca = torch.ones(cpy_n, dtype=torch.float32, device="cuda")
@@ -333,6 +347,7 @@ def main():
                       "batched_speedup": round(ms_loop / ms_step, 3), "kernel": best64,
                       "ms_by_kernel": {k: round(v, 4) for k, v in loop_ms.items()}},
         "copy_GBps": round(copy_gbps, 1),
+        "backward": {"ms": round(ms_bwd, 4), "pairs_per_s": round(nnz / (ms_bwd * 1e-3), 1)},
         "graph_ms_per_step": round(ms_graph, 5),
         "accuracy": {"fp32_max_rel_field_error": rel_err, "fp32_conservation_drift": drift,
                      "fp64_conservation_drift": drift64},

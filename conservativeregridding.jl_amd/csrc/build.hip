@@ -392,9 +392,10 @@ __global__ void __launch_bounds__(64) k_search(const float4 *__restrict__ qcap,
 }
 
 // Candidates whose subject lies outside one clip edge (about a quarter on the
-// benchmark grids) are settled by a cheap uniform test; the others are
-// queued in LDS and clipped 64 at a time, so every lane of the wave runs the
-// expensive pipeline on a real pair.  One wave per kClipChunks * 64
+// benchmark grids) or entirely inside the clip polygon (area = the subject's
+// area) are settled by a cheap uniform test; the others are queued in LDS and
+// clipped 64 at a time, so every lane of the wave runs the expensive pipeline
+// on a real pair.  One wave per kClipChunks * 64
 // candidates; results go to each candidate's own slot.  (Used with a positive
 // drop threshold: a pair the test settles could at most leave a sub-rounding
 // sliver in the pipeline, which the threshold drops too.)
@@ -440,8 +441,13 @@ __global__ void __launch_bounds__(64) k_clip_compact(const int32_t *__restrict__
                 const double *sp = src_cells + 3 * (int64_t)src_nv * s;
                 const double *clip = clip_is_dst ? dp : sp, *subj = clip_is_dst ? sp : dp;
                 const int nc = clip_is_dst ? dst_nv : src_nv, ns = clip_is_dst ? src_nv : dst_nv;
-                keep = !outside_one_edge<NC>(clip, nc, subj, ns);
-                if (!keep) val[c] = 0.0;
+                const int where = classify_subject<NC>(clip, nc, subj, ns);
+                keep = where == kSubjectCrosses;
+                if (where == kSubjectOutside) val[c] = 0.0;
+                if (where == kSubjectInside) {  // the subject's own (unit-sphere) area
+                    const double a = clip_is_dst ? a_src[s] : a_dst[d];
+                    val[c] = a > thr * fmin(a_src[s], a_dst[d]) ? a : 0.0;
+                }
             }
             const uint64_t m = __ballot(keep);
             if (keep) queue[qn + (int)__popcll(m & ((1ull << lane) - 1ull))] = ch * 64 + lane;

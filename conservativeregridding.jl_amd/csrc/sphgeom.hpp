@@ -189,11 +189,16 @@ CR_HD double clip_area_stream(const double *clip, int nc, const double *subj, in
     return P.acc.area();
 }
 
-// True when every subject vertex lies strictly outside the great circle of
-// one clip edge (the pipeline's own side test, n.(v - a) < 0; a degenerate
-// edge rejects nothing): the intersection is then empty.
+// Where the subject lies against the clip polygon, by the pipeline's own side
+// test n.(v - a) (a degenerate edge has n = 0: everything is on its inside):
+// kSubjectOutside when every subject vertex is strictly outside one clip edge
+// (the intersection is empty), kSubjectInside when every vertex is inside
+// every edge (the pipeline would pass the subject through unchanged: the
+// intersection is the subject), kSubjectCrosses otherwise.
+enum { kSubjectOutside = 0, kSubjectInside = 1, kSubjectCrosses = 2 };
+
 template <int NC>
-CR_HD bool outside_one_edge(const double *clip, int nc, const double *subj, int ns) {
+CR_HD int classify_subject(const double *clip, int nc, const double *subj, int ns) {
     d3 a[NC], n[NC];
 #pragma unroll
     for (int k = 0; k < NC; ++k) a[k] = load3(clip + 3 * (k < nc ? k : nc - 1));
@@ -201,18 +206,24 @@ CR_HD bool outside_one_edge(const double *clip, int nc, const double *subj, int 
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
         d3 nk = cross(a[k], a[(k + 1) % NC] - a[k]);
-        out[k] = dot(nk, nk) >= kDegenerateEdge2;
+        if (dot(nk, nk) < kDegenerateEdge2) nk = mk3(0.0, 0.0, 0.0);
+        out[k] = true;
         n[k] = nk;
     }
+    bool in = true;
     for (int i = 0; i < ns; ++i) {
         const d3 v = load3(subj + 3 * i);
 #pragma unroll
-        for (int k = 0; k < NC; ++k) out[k] = out[k] && dot(n[k], v - a[k]) < 0.0;
+        for (int k = 0; k < NC; ++k) {
+            const double s = dot(n[k], v - a[k]);
+            out[k] = out[k] && s < 0.0;
+            in = in && s >= 0.0;
+        }
     }
     bool any = false;
 #pragma unroll
     for (int k = 0; k < NC; ++k) any = any || out[k];
-    return any;
+    return any ? kSubjectOutside : (in ? kSubjectInside : kSubjectCrosses);
 }
 
 // Signed area of an open ring of n vertices.
