@@ -7,7 +7,8 @@ Loaded from its folder by path (the folder name is not a Python identifier):
 from ._lib import CRError, LIB_PATH, lib as _lib_handle
 from .errors import ArgumentError, DimensionMismatch, MethodError
 from .grids import (Grid, NESTED, RING, as_grid, cubed_sphere_points, healpix_cells,
-                    healpix_nest2ring, healpix_ring2nest, lonlat_points, points_to_cells)
+                    flatten_tree, healpix_nest2ring, healpix_ring2nest, lonlat_points, lonlat_to_unit, points_to_cells,
+                    polygons_to_cells, polygons_width, ring_full_polygons, ring_latitudes, ring_longitudes)
 from .esmf import save_esmf_weights, write_esmf_arrays
 from .regridder import (EARTH_RADIUS, Regridder, areas, normalize_, regrid, regrid_, regrid_plan,
                         regrid_tune, regrid_backward, regrid_variants, regrid_set_variant,
@@ -17,7 +18,8 @@ __all__ = [
     "Regridder", "regrid_", "regrid", "regrid_plan", "regrid_tune", "regrid_backward", "regrid_variants",
     "regrid_set_variant", "transpose", "normalize_", "areas", "size", "save_esmf_weights", "write_esmf_arrays",
     "Grid", "NESTED", "RING", "as_grid", "lonlat_points", "cubed_sphere_points", "healpix_cells",
-    "healpix_nest2ring", "healpix_ring2nest", "points_to_cells",
+    "healpix_nest2ring", "healpix_ring2nest", "points_to_cells", "lonlat_to_unit", "polygons_to_cells",
+    "polygons_width", "flatten_tree", "ring_full_polygons", "ring_latitudes", "ring_longitudes",
     "ArgumentError", "DimensionMismatch", "MethodError", "CRError", "EARTH_RADIUS", "LIB_PATH",
     "device_count", "version",
 ]

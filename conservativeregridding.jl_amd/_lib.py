@@ -60,11 +60,15 @@ SIGNATURES = {
     "cr_device_count": (c_i32, []),
     "cr_gen_lonlat_points": (c_i32, [_dp, c_i64, c_i64, c_dbl, c_dbl, c_dbl, c_dbl]),
     "cr_gen_cubed_sphere_points": (c_i32, [_dp, c_i64]),
+    "cr_lonlat_to_xyz": (c_i32, [_dp, _dp, c_i64, _dp]),
     "cr_gen_healpix_cells": (c_i32, [_dp, c_i64, c_i32, c_i32, c_i64, c_i64]),
     "cr_healpix_nest2ring": (c_i32, [c_i64, c_i64, _i64p]),
     "cr_healpix_ring2nest": (c_i32, [c_i64, c_i64, _i64p]),
     "cr_points_to_cells": (c_i32, [_dp, c_i64, c_i64, c_i64, _dp]),
+    "cr_polygons_width": (c_i32, [c_i64, _i64p, _i32p, c_i64, _i32p]),
+    "cr_polygons_to_cells": (c_i32, [_dp, c_i64, _i64p, _i32p, c_i64, c_i32, _dp, _i32p]),
     "cr_grid_from_cells": (c_i32, [ctypes.POINTER(c_vp), _dp, c_i64, c_i32, c_i32, c_vp]),
+    "cr_grid_from_polygons": (c_i32, [ctypes.POINTER(c_vp), _dp, c_i64, _i64p, _i32p, c_i64, c_i32, c_vp]),
     "cr_grid_from_points": (c_i32, [ctypes.POINTER(c_vp), _dp, c_i64, c_i64, c_i64, c_vp]),
     "cr_grid_healpix": (c_i32, [ctypes.POINTER(c_vp), c_i64, c_i32, c_i32, c_vp]),
     "cr_grid_info": (c_i32, [c_vp, _i64p, _i32p, _i32p]),

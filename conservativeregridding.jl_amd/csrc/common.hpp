@@ -121,6 +121,9 @@ inline void check_device() {
     if (e != hipSuccess || n == 0) throw Error{CR_NO_DEVICE, "no HIP device is visible"};
 }
 
+// Widest cell polygon a grid may hold (vertices per cell).
+constexpr int kMaxCellVertices = 64;
+
 inline unsigned grid_for(int64_t n, int block) {
     int64_t g = (n + block - 1) / block;
     return (unsigned)(g < 1 ? 1 : g);

@@ -1,5 +1,6 @@
 // grids.hip -- grid generators (host entry points) and device grid creation.
 #include <algorithm>
+#include <atomic>
 #include <thread>
 #include <vector>
 
@@ -101,6 +102,53 @@ void parallel_for(int64_t n, F &&f) {
     for (auto &x : th) x.join();
 }
 
+// Vertex count of polygon c of a ragged list; a ring whose last index repeats
+// its first is closed (the GI.LinearRing convention of Trees.getcell) and
+// loses the repeat.
+int64_t polygon_len(const int64_t *off, const int32_t *verts, int64_t c) {
+    const int64_t a = off[c], b = off[c + 1];
+    int64_t n = b - a;
+    if (n >= 2 && verts[b - 1] == verts[a]) --n;
+    return n;
+}
+
+// Validate a ragged polygon list and return its widest polygon.
+int32_t polygons_width(int64_t nverts, const int64_t *off, const int32_t *verts, int64_t ncells) {
+    CR_REQUIRE(off[0] == 0, CR_INVALID_ARGUMENT, "polygon offsets must start at 0");
+    int64_t width = 0;
+    for (int64_t c = 0; c < ncells; ++c) {
+        CR_REQUIRE(off[c + 1] >= off[c], CR_INVALID_ARGUMENT, "polygon offsets must not decrease");
+        const int64_t n = polygon_len(off, verts, c);
+        CR_REQUIRE(n >= 3 && n <= kMaxCellVertices, CR_INVALID_ARGUMENT,
+                   "polygon " + std::to_string(c) + " has " + std::to_string(n) +
+                       " vertices (need 3.." + std::to_string(kMaxCellVertices) + ")");
+        width = std::max(width, n);
+    }
+    std::atomic<int64_t> bad{-1};
+    parallel_for(off[ncells], [&](int64_t k) {
+        if (verts[k] < 0 || verts[k] >= nverts) bad.store(k);
+    });
+    CR_REQUIRE(bad.load() < 0, CR_INVALID_ARGUMENT,
+               "polygon vertex index out of range at position " + std::to_string(bad.load()));
+    return (int32_t)width;
+}
+
+// Fixed-width cell table of a ragged polygon list: polygon c occupies slots
+// [c*nv, (c+1)*nv); a polygon with fewer than nv vertices repeats its last
+// vertex.  The repeats are zero-length edges, which the clip (zero normal:
+// every point is inside), the area fan (zero triangles) and the caps ignore.
+void polygons_fill(const double *xyz, const int64_t *off, const int32_t *verts, int64_t ncells, int nv,
+                   double *cells, int32_t *vidx) {
+    parallel_for(ncells, [&](int64_t c) {
+        const int64_t n = polygon_len(off, verts, c);
+        for (int k = 0; k < nv; ++k) {
+            const int32_t v = verts[off[c] + std::min<int64_t>(k, n - 1)];
+            for (int d = 0; d < 3; ++d) cells[(c * nv + k) * 3 + d] = xyz[3 * (int64_t)v + d];
+            if (vidx) vidx[c * nv + k] = v;
+        }
+    });
+}
+
 }  // namespace cr
 
 using namespace cr;
@@ -119,6 +167,13 @@ cr_status cr_gen_lonlat_points(double *xyz, int64_t ni, int64_t nj, double lon0,
             int64_t j = t / W, i = t - j * W;
             lonlat_to_xyz(lon0 + (double)i * dlon, lat0 + (double)j * dlat, xyz + 3 * t);
         });
+    });
+}
+
+cr_status cr_lonlat_to_xyz(const double *lon_deg, const double *lat_deg, int64_t n, double *xyz) {
+    return guard([&] {
+        CR_REQUIRE(lon_deg && lat_deg && xyz && n >= 0, CR_INVALID_ARGUMENT, "cr_lonlat_to_xyz: bad arguments");
+        parallel_for(n, [&](int64_t t) { lonlat_to_xyz(lon_deg[t], lat_deg[t], xyz + 3 * t); });
     });
 }
 
@@ -198,7 +253,7 @@ cr_status cr_points_to_cells(const double *pts, int64_t nblocks, int64_t ni, int
 cr_status cr_grid_from_cells(cr_grid **grid, const double *cells, int64_t ncells, int32_t nv,
                              int32_t convex, void *stream) {
     return guard([&] {
-        CR_REQUIRE(grid && cells && ncells > 0 && nv >= 3 && nv <= 64, CR_INVALID_ARGUMENT,
+        CR_REQUIRE(grid && cells && ncells > 0 && nv >= 3 && nv <= kMaxCellVertices, CR_INVALID_ARGUMENT,
                    "cr_grid_from_cells: need ncells > 0 and 3 <= nv <= 64");
         CR_REQUIRE(ncells < (int64_t(1) << 31), CR_UNSUPPORTED, "more than 2^31-1 cells");
         check_device();
@@ -216,6 +271,62 @@ cr_status cr_grid_from_cells(cr_grid **grid, const double *cells, int64_t ncells
                                g->nverts);
             orient_grid(g, s);
         } catch (...) {
+            delete g;
+            throw;
+        }
+        *grid = g;
+    });
+}
+
+cr_status cr_polygons_to_cells(const double *xyz, int64_t nverts, const int64_t *offsets,
+                               const int32_t *verts, int64_t ncells, int32_t nv, double *cells,
+                               int32_t *vidx) {
+    return guard([&] {
+        CR_REQUIRE(xyz && offsets && verts && cells && nverts > 0 && ncells > 0, CR_INVALID_ARGUMENT,
+                   "cr_polygons_to_cells: bad arguments");
+        const int32_t width = polygons_width(nverts, offsets, verts, ncells);
+        CR_REQUIRE(nv >= width && nv <= kMaxCellVertices, CR_INVALID_ARGUMENT,
+                   "cr_polygons_to_cells: nv is below the widest polygon (" + std::to_string(width) + ")");
+        polygons_fill(xyz, offsets, verts, ncells, nv, cells, vidx);
+    });
+}
+
+cr_status cr_polygons_width(int64_t nverts, const int64_t *offsets, const int32_t *verts, int64_t ncells,
+                            int32_t *nv) {
+    return guard([&] {
+        CR_REQUIRE(offsets && verts && nv && nverts > 0 && ncells > 0, CR_INVALID_ARGUMENT,
+                   "cr_polygons_width: bad arguments");
+        *nv = polygons_width(nverts, offsets, verts, ncells);
+    });
+}
+
+cr_status cr_grid_from_polygons(cr_grid **grid, const double *xyz, int64_t nverts, const int64_t *offsets,
+                                const int32_t *verts, int64_t ncells, int32_t convex, void *stream) {
+    return guard([&] {
+        CR_REQUIRE(grid && xyz && offsets && verts && nverts > 0 && ncells > 0, CR_INVALID_ARGUMENT,
+                   "cr_grid_from_polygons: bad arguments");
+        CR_REQUIRE(ncells < (int64_t(1) << 31) && nverts < (int64_t(1) << 31), CR_UNSUPPORTED,
+                   "more than 2^31-1 cells or vertices");
+        check_device();
+        const int32_t nv = polygons_width(nverts, offsets, verts, ncells);
+        std::vector<double> cells((size_t)ncells * nv * 3);
+        std::vector<int32_t> vidx((size_t)ncells * nv);
+        polygons_fill(xyz, offsets, verts, ncells, nv, cells.data(), vidx.data());
+        hipStream_t s = as_stream(stream);
+        auto *g = new cr_grid();
+        try {
+            g->ncells = ncells;
+            g->nv = nv;
+            g->convex = convex ? 1 : 0;
+            g->nverts = nverts;
+            g->cells.alloc(cells.size());
+            g->vidx.alloc(vidx.size());
+            CR_HIP(hipMemcpyAsync(g->cells.p, cells.data(), g->cells.bytes(), hipMemcpyHostToDevice, s));
+            CR_HIP(hipMemcpyAsync(g->vidx.p, vidx.data(), g->vidx.bytes(), hipMemcpyHostToDevice, s));
+            CR_HIP(hipStreamSynchronize(s));  // the host tables end with this scope
+            orient_grid(g, s);
+        } catch (...) {
+            hipStreamSynchronize(s);
             delete g;
             throw;
         }

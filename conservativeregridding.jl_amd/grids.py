@@ -11,6 +11,11 @@ Generators (no GPU needed; they run in libcrhip.so's host code):
 * :func:`cubed_sphere_points` -- 6 equiangular gnomonic faces, ``(6, ne+1, ne+1, 3)``
 * :func:`healpix_cells`  -- HEALPix pixel boundaries with ``step`` points per side
 * :func:`points_to_cells` -- CellBasedGrid ``getcell`` order (src/trees/grids.jl:71-84)
+* :func:`polygons_to_cells` -- ragged polygon lists (any grid given by
+  ``Trees.ncells`` / ``Trees.getcell``) padded to a fixed-width table
+* :func:`ring_full_polygons` -- the cells of a RingGrids full grid
+  (ext/ConservativeRegriddingRingGridsExt.jl:22-50)
+* :func:`lonlat_to_unit` -- degree-exact geographic -> unit vectors
 """
 from __future__ import annotations
 
@@ -90,6 +95,134 @@ def _point_blocks(points) -> tuple[np.ndarray, int, int, int]:
     return c, nb, W - 1, H - 1
 
 
+def lonlat_to_unit(lon_deg, lat_deg) -> np.ndarray:
+    """Unit vectors of (lon, lat) in degrees, shape ``broadcast.shape + (3,)``;
+    exact at multiples of 90 degrees (``UnitSphereFromGeographic``)."""
+    lon, lat = np.broadcast_arrays(np.asarray(lon_deg, dtype=np.float64), np.asarray(lat_deg, dtype=np.float64))
+    lon, lat = np.ascontiguousarray(lon), np.ascontiguousarray(lat)
+    out = np.empty(lon.shape + (3,))
+    L.check(L.lib.cr_lonlat_to_xyz(L.dptr(lon), L.dptr(lat), lon.size, L.dptr(out)), "lonlat_to_unit")
+    return out
+
+
+def _ragged(vertices, offsets, indices):
+    v = np.ascontiguousarray(vertices, dtype=np.float64)
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    idx = np.ascontiguousarray(indices, dtype=np.int32)
+    if v.ndim != 2 or v.shape[1] != 3 or v.shape[0] == 0:
+        raise ArgumentError("vertices must have shape (nverts, 3)")
+    if off.ndim != 1 or off.size < 2 or idx.ndim != 1 or off[-1] != idx.size:
+        raise ArgumentError("offsets must be (ncells+1,) with offsets[-1] == len(indices)")
+    return v, off, idx
+
+
+def polygons_width(vertices, offsets, indices) -> int:
+    """Widest polygon (vertex count, closing repeats dropped) of a ragged list."""
+    v, off, idx = _ragged(vertices, offsets, indices)
+    nv = ctypes.c_int32()
+    L.check(L.lib.cr_polygons_width(v.shape[0], L.i64ptr(off), L.i32ptr(idx), off.size - 1, ctypes.byref(nv)),
+            "polygons_width")
+    return nv.value
+
+
+def polygons_to_cells(vertices, offsets, indices, nv: int | None = None):
+    """Fixed-width table of a ragged polygon list: ``(cells (ncells, nv, 3),
+    vidx (ncells, nv))``.  Cell c has vertices ``indices[offsets[c]:offsets[c+1]]``
+    (a repeated closing index is dropped); narrower cells repeat their last vertex."""
+    v, off, idx = _ragged(vertices, offsets, indices)
+    nc = off.size - 1
+    nv = polygons_width(v, off, idx) if nv is None else int(nv)
+    cells = np.empty((nc, nv, 3))
+    vidx = np.empty((nc, nv), dtype=np.int32)
+    L.check(L.lib.cr_polygons_to_cells(L.dptr(v), v.shape[0], L.i64ptr(off), L.i32ptr(idx), nc, nv,
+                                       L.dptr(cells), L.i32ptr(vidx)), "polygons_to_cells")
+    return cells, vidx
+
+
+def ring_full_polygons(latd, lond):
+    """Cells of a RingGrids full grid with ring latitudes ``latd`` (degrees, north
+    to south) and longitudes ``lond`` (cell centres, equally spaced), as a ragged
+    polygon list ``(vertices, offsets, indices)``.
+
+    Follows treeify for ``AbstractFullGrid`` (ext/ConservativeRegriddingRingGridsExt.jl:22-50):
+    latitude edges are the midpoints between rings, pinned to +-90 at the poles;
+    longitude edges sit half a cell west of each centre; the vertex matrix is
+    ``P[i, j] = (lon_edges[i], lat_edges[nlat+1-j])`` (south to north in j) and
+    cell ``k = ring*nlon + i`` (rings north to south, 0-based) is cartesian cell
+    ``(i, nlat-1-ring)`` of the CellBasedGrid, the ``Reorderer2D`` of the extension."""
+    latd = np.asarray(latd, dtype=np.float64)
+    lond = np.asarray(lond, dtype=np.float64)
+    nlat, nlon = latd.size, lond.size
+    if nlat < 1 or nlon < 2:
+        raise ArgumentError("a full ring grid needs at least one ring and two longitudes")
+    lat_edges = np.empty(nlat + 1)
+    lat_edges[0], lat_edges[-1] = 90.0, -90.0
+    lat_edges[1:-1] = 0.5 * (latd[:-1] + latd[1:])
+    dlon = 360.0 / nlon
+    lon_edges = lond[0] - dlon / 2 + np.arange(nlon + 1) * dlon
+    W = nlon + 1
+    jj, ii = np.meshgrid(np.arange(nlat + 1), np.arange(W), indexing="ij")  # vertex id ii + jj*W
+    verts = lonlat_to_unit(lon_edges[ii], lat_edges[nlat - jj]).reshape(-1, 3)
+    ring, i = np.meshgrid(np.arange(nlat), np.arange(nlon), indexing="ij")
+    j = nlat - 1 - ring
+    quad = np.stack([i + j * W, i + 1 + j * W, i + 1 + (j + 1) * W, i + (j + 1) * W], axis=-1)
+    idx = quad.reshape(-1).astype(np.int32)
+    off = np.arange(0, idx.size + 1, 4, dtype=np.int64)
+    return verts, off, idx
+
+
+def ring_latitudes(kind: str, nlat_half: int) -> np.ndarray:
+    """Ring latitudes (degrees, north to south) of RingGrids' full grids:
+    ``"gaussian"`` -- 2*nlat_half Gauss-Legendre latitudes (FullGaussianGrid);
+    ``"clenshaw"`` -- 2*nlat_half-1 equidistant latitudes without the poles,
+    colatitudes j*180/(nlat+1) (FullClenshawGrid)."""
+    if nlat_half < 1:
+        raise ArgumentError("nlat_half must be positive")
+    if kind == "gaussian":
+        mu, _ = np.polynomial.legendre.leggauss(2 * nlat_half)
+        return np.degrees(np.arcsin(mu[::-1]))
+    if kind == "clenshaw":
+        nlat = 2 * nlat_half - 1
+        return 90.0 - np.arange(1, nlat + 1) * 180.0 / (nlat + 1)
+    raise ArgumentError(f"unknown full ring grid {kind!r}")
+
+
+def ring_longitudes(nlat_half: int) -> np.ndarray:
+    """Cell-centre longitudes (degrees) of RingGrids' full grids: 4*nlat_half from 0."""
+    nlon = 4 * nlat_half
+    return np.arange(nlon) * (360.0 / nlon)
+
+
+def flatten_tree(tree, share_vertices: bool = True):
+    """Ragged polygon list ``(vertices, offsets, indices)`` of any object with the
+    Trees interface: ``ncells`` (method or attribute) and ``getcell(i)`` (0-based,
+    a ring of unit vectors ``(k, 3)`` or lon/lat degrees ``(k, 2)``, open or
+    closed).  With ``share_vertices`` bitwise-equal vertices become one."""
+    n = tree.ncells() if callable(getattr(tree, "ncells", None)) else int(tree.ncells)
+    if n <= 0:
+        raise ArgumentError("tree has no cells")
+    rings = []
+    for i in range(n):
+        r = np.asarray(tree.getcell(i), dtype=np.float64)
+        if r.ndim != 2 or r.shape[1] not in (2, 3):
+            raise ArgumentError(f"getcell({i}) must return (k, 3) unit vectors or (k, 2) lon/lat degrees")
+        if r.shape[1] == 2:
+            r = lonlat_to_unit(r[:, 0], r[:, 1])
+        if r.shape[0] >= 2 and np.array_equal(r[0], r[-1]):
+            r = r[:-1]
+        rings.append(r)
+    lens = np.fromiter((r.shape[0] for r in rings), dtype=np.int64, count=n)
+    off = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(lens, out=off[1:])
+    allv = np.concatenate(rings)
+    if share_vertices:
+        verts, inv = np.unique(allv, axis=0, return_inverse=True)
+        idx = inv.reshape(-1).astype(np.int32)
+    else:
+        verts, idx = allv, np.arange(allv.shape[0], dtype=np.int32)
+    return verts, off, idx
+
+
 def points_to_cells(points) -> np.ndarray:
     """Polygon table ``(ncells, 4, 3)`` of point blocks, cell index
     ``b*ni*nj + i + j*ni`` with vertices P[i,j], P[i+1,j], P[i+1,j+1], P[i,j+1]."""
@@ -141,6 +274,52 @@ class Grid:
         return cls(h.value, (c.shape[0],), "cells")
 
     @classmethod
+    def from_polygons(cls, vertices, offsets, indices, convex: bool = True, stream=None) -> "Grid":
+        """Ragged polygon list with shared vertices: cell c is the ring
+        ``vertices[indices[offsets[c]:offsets[c+1]]]`` (3..64 vertices; a repeated
+        closing index is dropped).  Vertex gradients are indexed by ``vertices``."""
+        v, off, idx = _ragged(vertices, offsets, indices)
+        h = ctypes.c_void_p()
+        L.check(L.lib.cr_grid_from_polygons(ctypes.byref(h), L.dptr(v), v.shape[0], L.i64ptr(off),
+                                            L.i32ptr(idx), off.size - 1, 1 if convex else 0, _stream(stream)),
+                "Grid.from_polygons")
+        return cls(h.value, (off.size - 1,), "polygons")
+
+    @classmethod
+    def from_tree(cls, tree, convex: bool = True, share_vertices: bool = True, stream=None) -> "Grid":
+        """The generic tree adapter: flatten any object with the Trees interface,
+        ``ncells`` (method or attribute) and ``getcell(i)`` (0-based, a ring of
+        unit vectors ``(k, 3)`` or of lon/lat degrees ``(k, 2)``, open or closed),
+        as ``Trees.treeify`` + ``Trees.getcell`` do for the reference's grids
+        (src/trees/interfaces.jl:94-129, docs/src/interfaces/trees.md:34-44).
+        With ``share_vertices`` bitwise-equal vertices become one vertex, so
+        vertex gradients sum over the cells that share it."""
+        verts, off, idx = flatten_tree(tree, share_vertices)
+        g = cls.from_polygons(verts, off, idx, convex=convex, stream=stream)
+        g.kind = "tree"
+        return g
+
+    @classmethod
+    def from_lonlat_cells(cls, cells_deg, convex: bool = True, stream=None) -> "Grid":
+        """Polygon table of lon/lat degrees ``(ncells, k, 2)`` (e.g. the E,S,W,N
+        faces of a reduced ring grid, open or closed rings, either orientation)."""
+        c = np.asarray(cells_deg, dtype=np.float64)
+        if c.ndim != 3 or c.shape[2] != 2:
+            raise ArgumentError("cells must have shape (ncells, k, 2)")
+        if c.shape[1] >= 4 and np.array_equal(c[:, 0], c[:, -1]):
+            c = c[:, :-1]
+        return cls.from_cells(lonlat_to_unit(c[..., 0], c[..., 1]), convex=convex, stream=stream)
+
+    @classmethod
+    def ring_full(cls, latd, lond, stream=None) -> "Grid":
+        """A RingGrids full grid (``treeify(::Spherical, ::AbstractFullGrid)``):
+        cells ring by ring from the north, see :func:`ring_full_polygons`."""
+        latd = np.asarray(latd, dtype=np.float64)
+        g = cls.from_polygons(*ring_full_polygons(latd, lond), convex=True, stream=stream)
+        g.kind = "ringgrid"
+        return g
+
+    @classmethod
     def lonlat(cls, ni: int, nj: int, **kw) -> "Grid":
         return cls.from_points(lonlat_points(ni, nj, **kw))
 
@@ -190,9 +369,12 @@ def _stream(stream):
 
 
 def as_grid(g) -> Grid:
-    """treeify: accept a Grid, a point matrix ``(ni+1, nj+1, 3)`` or point blocks."""
+    """treeify: accept a Grid, an object with the Trees interface (``ncells`` and
+    ``getcell``), a point matrix ``(ni+1, nj+1, 3)`` or point blocks."""
     if isinstance(g, Grid):
         return g
+    if hasattr(g, "getcell") and hasattr(g, "ncells"):
+        return Grid.from_tree(g)
     a = np.asarray(g)
     if a.dtype.kind == "f" and a.shape[-1] == 3 and a.ndim in (3, 4):
         return Grid.from_points(a)

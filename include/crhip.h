@@ -97,6 +97,10 @@ cr_status cr_gen_lonlat_points(double *xyz, int64_t ni, int64_t nj, double lon0_
  * Face f covers cells f*ne^2 + i + j*ne (the offsets of
  * ext/ConservativeRegriddingClimaCoreExt.jl:56-63). */
 cr_status cr_gen_cubed_sphere_points(double *xyz, int64_t ne);
+/* Unit vectors of n (lon, lat) degree pairs, exact at multiples of 90 degrees
+ * (GeometryOps UnitSphereFromGeographic, as used by
+ * ext/ConservativeRegriddingRingGridsExt.jl:42-44). */
+cr_status cr_lonlat_to_xyz(const double *lon_deg, const double *lat_deg, int64_t n, double *xyz);
 /* HEALPix cells [first, first+count) as 4*step-vertex polygons, counter-
  * clockwise from the north corner, in nested or ring index order.  Replaces
  * Healpix.boundariesRing (ext/ConservativeRegriddingHealpixExt.jl:76-90). */
@@ -110,6 +114,21 @@ cr_status cr_healpix_ring2nest(int64_t nside, int64_t ring, int64_t *nest);
  * P[i+1,j+1], P[i,j+1] (CellBasedGrid getcell, grids.jl:71-84). */
 cr_status cr_points_to_cells(const double *pts, int64_t nblocks, int64_t ni, int64_t nj,
                              double *cells);
+/* Ragged polygon lists, the flattened form of any grid that implements
+ * Trees.ncells / Trees.getcell (docs/src/interfaces/trees.md:34-44; RingGrids,
+ * ISEA and user grids): nverts unit vectors xyz (3 per vertex) shared by the
+ * cells, and cell c = vertices verts[offsets[c] .. offsets[c+1]-1] in ring
+ * order.  A ring whose last index repeats its first is closed and loses the
+ * repeat; every cell needs 3..64 vertices.  cr_polygons_width returns the
+ * widest cell; cr_polygons_to_cells writes the fixed-width table of nv >=
+ * that width (ncells*nv*3 doubles, narrower cells repeat their last vertex,
+ * a zero-length edge that clip and area ignore) and, if vidx is not NULL, the
+ * vertex index of every slot. */
+cr_status cr_polygons_width(int64_t nverts, const int64_t *offsets, const int32_t *verts,
+                            int64_t ncells, int32_t *nv);
+cr_status cr_polygons_to_cells(const double *xyz, int64_t nverts, const int64_t *offsets,
+                               const int32_t *verts, int64_t ncells, int32_t nv, double *cells,
+                               int32_t *vidx);
 
 /* ------------------------------------------------------ device grids -- */
 /* Upload a host polygon table.  `convex` declares every cell convex. */
@@ -119,6 +138,13 @@ cr_status cr_grid_from_cells(cr_grid **grid, const double *cells, int64_t ncells
  * cubed sphere six) and expand them to cells on the device. */
 cr_status cr_grid_from_points(cr_grid **grid, const double *pts, int64_t nblocks, int64_t ni,
                               int64_t nj, void *stream);
+/* Upload a ragged polygon list (layout of cr_polygons_to_cells) with shared
+ * vertices: the generic tree adapter (Trees.treeify of a grid given by
+ * Trees.ncells / Trees.getcell, src/trees/interfaces.jl:94-129).  Vertex
+ * gradients are indexed by the nverts shared vertices. */
+cr_status cr_grid_from_polygons(cr_grid **grid, const double *xyz, int64_t nverts,
+                                const int64_t *offsets, const int32_t *verts, int64_t ncells,
+                                int32_t convex, void *stream);
 /* Generate a HEALPix grid on the device (HealpixRootNode{O}(nside)). */
 cr_status cr_grid_healpix(cr_grid **grid, int64_t nside, int32_t order, int32_t step,
                           void *stream);
