@@ -71,6 +71,10 @@ SIGNATURES = {
     "cr_grid_from_polygons": (c_i32, [ctypes.POINTER(c_vp), _dp, c_i64, _i64p, _i32p, c_i64, c_i32, c_vp]),
     "cr_grid_from_points": (c_i32, [ctypes.POINTER(c_vp), _dp, c_i64, c_i64, c_i64, c_vp]),
     "cr_grid_healpix": (c_i32, [ctypes.POINTER(c_vp), c_i64, c_i32, c_i32, c_vp]),
+    "cr_grid_tripolar": (c_i32, [ctypes.POINTER(c_vp), _dp, c_i64, c_i64, c_vp]),
+    "cr_grid_fold_count": (c_i32, [c_vp, _i64p]),
+    "cr_mirror_fold_partners": (c_i32, [c_vp, c_i32, c_i32, c_i64, c_vp, c_i64, c_i64, c_vp]),
+    "cr_mirror_fold_partners_adjoint": (c_i32, [c_vp, c_i32, c_i64, c_vp, c_i64, c_i64, c_vp]),
     "cr_grid_info": (c_i32, [c_vp, _i64p, _i32p, _i32p]),
     "cr_grid_vertex_count": (c_i32, [c_vp, _i64p]),
     "cr_grid_copy_cells": (c_i32, [c_vp, _dp]),

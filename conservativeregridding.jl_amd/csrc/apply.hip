@@ -351,11 +351,71 @@ void apply(const cr_regridder *R, bool transpose, int dtype, int64_t nf, const v
     CR_HIP(hipGetLastError());
 }
 
+// Fold partners: y[partner] = y[primary] for every field (the mirror kernel of
+// ext/ConservativeRegriddingOceananigansExt.jl:223-240), and its adjoint
+// g[primary] += g[partner], g[partner] = 0 (each partner has one primary).
+template <class T>
+__global__ void k_mirror(T *__restrict__ y, int64_t yc, int64_t yf, int64_t nf, const int32_t *__restrict__ pairs,
+                         int64_t npairs) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npairs * nf) return;
+    const int64_t f = t / npairs, k = t - f * npairs;
+    y[pairs[2 * k + 1] * yc + f * yf] = y[pairs[2 * k] * yc + f * yf];
+}
+
+__global__ void k_mirror_adjoint(double *__restrict__ g, int64_t gc, int64_t gf, int64_t nf,
+                                 const int32_t *__restrict__ pairs, int64_t npairs) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npairs * nf) return;
+    const int64_t f = t / npairs, k = t - f * npairs;
+    double &gp = g[pairs[2 * k + 1] * gc + f * gf];
+    g[pairs[2 * k] * gc + f * gf] += gp;
+    gp = 0.0;
+}
+
 }  // namespace cr
 
 using namespace cr;
 
 extern "C" {
+
+cr_status cr_mirror_fold_partners(const cr_regridder *R, int32_t transpose, int32_t dtype, int64_t nfields,
+                                  void *dst, int64_t dst_cell_stride, int64_t dst_field_stride, void *stream) {
+    return guard([&] {
+        CR_REQUIRE(R, CR_INVALID_ARGUMENT, "cr_mirror_fold_partners: null regridder");
+        CR_REQUIRE(dtype == CR_F64 || dtype == CR_F32, CR_UNSUPPORTED, "dtype must be F64 or F32");
+        CR_REQUIRE(nfields >= 0 && dst_cell_stride > 0 && (nfields <= 1 || dst_field_stride > 0),
+                   CR_INVALID_ARGUMENT, "cr_mirror_fold_partners: bad sizes or strides");
+        const int64_t n = transpose ? R->nfold_src : R->nfold_dst;
+        if (n == 0 || nfields == 0) return;
+        CR_REQUIRE(dst, CR_INVALID_ARGUMENT, "cr_mirror_fold_partners: null field pointer");
+        const int32_t *pairs = transpose ? R->fold_src.p : R->fold_dst.p;
+        hipStream_t s = as_stream(stream);
+        const unsigned g = grid_for(n * nfields, 256);
+        if (dtype == CR_F64)
+            hipLaunchKernelGGL(k_mirror<double>, dim3(g), dim3(256), 0, s, (double *)dst, dst_cell_stride,
+                               dst_field_stride, nfields, pairs, n);
+        else
+            hipLaunchKernelGGL(k_mirror<float>, dim3(g), dim3(256), 0, s, (float *)dst, dst_cell_stride,
+                               dst_field_stride, nfields, pairs, n);
+        CR_HIP(hipGetLastError());
+    });
+}
+
+cr_status cr_mirror_fold_partners_adjoint(const cr_regridder *R, int32_t transpose, int64_t nfields, double *g,
+                                          int64_t cell_stride, int64_t field_stride, void *stream) {
+    return guard([&] {
+        CR_REQUIRE(R, CR_INVALID_ARGUMENT, "cr_mirror_fold_partners_adjoint: null regridder");
+        CR_REQUIRE(nfields >= 0 && cell_stride > 0 && (nfields <= 1 || field_stride > 0), CR_INVALID_ARGUMENT,
+                   "cr_mirror_fold_partners_adjoint: bad sizes or strides");
+        const int64_t n = transpose ? R->nfold_src : R->nfold_dst;
+        if (n == 0 || nfields == 0) return;
+        CR_REQUIRE(g, CR_INVALID_ARGUMENT, "cr_mirror_fold_partners_adjoint: null gradient pointer");
+        hipLaunchKernelGGL(k_mirror_adjoint, dim3(grid_for(n * nfields, 256)), dim3(256), 0, as_stream(stream), g,
+                           cell_stride, field_stride, nfields, transpose ? R->fold_src.p : R->fold_dst.p, n);
+        CR_HIP(hipGetLastError());
+    });
+}
 
 cr_status cr_regrid(const cr_regridder *R, int32_t transpose, int32_t dtype, int64_t nfields,
                     const void *src, int64_t src_cell_stride, int64_t src_field_stride, void *dst,

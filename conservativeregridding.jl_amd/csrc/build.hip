@@ -401,6 +401,15 @@ __global__ void __launch_bounds__(64) k_search(const float4 *__restrict__ qcap,
 // sliver in the pipeline, which the threshold drops too.)
 constexpr int kClipChunks = 8;
 
+// The kept value of a pair: its area if above threshold_rel times the smaller
+// cell area, else 0.  A zero-area cell (a padding ghost such as the duplicated
+// half of a tripolar fold row, ext/ConservativeRegriddingOceananigansExt.jl:65-115,
+// never a tree candidate there) intersects nothing: as a clip polygon its
+// zero-length edges would pass the whole subject.
+__device__ __forceinline__ double kept_area(double a, double amin, double thr) {
+    return amin > 0.0 && a > thr * amin ? a : 0.0;
+}
+
 template <int NC>
 __device__ __forceinline__ void clip_candidate(int64_t c, const int32_t *__restrict__ cand_dst,
                                                const int32_t *__restrict__ cand_src,
@@ -415,8 +424,7 @@ __device__ __forceinline__ void clip_candidate(int64_t c, const int32_t *__restr
     const double *clip = clip_is_dst ? dp : sp, *subj = clip_is_dst ? sp : dp;
     const int nc = clip_is_dst ? dst_nv : src_nv, ns = clip_is_dst ? src_nv : dst_nv;
     const double a = clip_area_stream<NC>(clip, nc, subj, ns);
-    const double amin = fmin(a_src[s], a_dst[d]);
-    val[c] = a > thr * amin ? a : 0.0;
+    val[c] = kept_area(a, fmin(a_src[s], a_dst[d]), thr);
 }
 
 template <int NC>
@@ -446,7 +454,7 @@ __global__ void __launch_bounds__(64) k_clip_compact(const int32_t *__restrict__
                 if (where == kSubjectOutside) val[c] = 0.0;
                 if (where == kSubjectInside) {  // the subject's own (unit-sphere) area
                     const double a = clip_is_dst ? a_src[s] : a_dst[d];
-                    val[c] = a > thr * fmin(a_src[s], a_dst[d]) ? a : 0.0;
+                    val[c] = kept_area(a, fmin(a_src[s], a_dst[d]), thr);
                 }
             }
             const uint64_t m = __ballot(keep);
@@ -486,8 +494,7 @@ __global__ void __launch_bounds__(256) k_clip(const int32_t *__restrict__ cand_d
     const double *clip = clip_is_dst ? dp : sp, *subj = clip_is_dst ? sp : dp;
     const int nc = clip_is_dst ? dst_nv : src_nv, ns = clip_is_dst ? src_nv : dst_nv;
     double a = clip_area_stream<NC>(clip, nc, subj, ns);
-    double amin = fmin(a_src[s], a_dst[d]);
-    val[c] = a > thr * amin ? a : 0.0;
+    val[c] = kept_area(a, fmin(a_src[s], a_dst[d]), thr);
 }
 
 // Generic clip for convex clip polygons with more than 4 vertices: classic
@@ -542,8 +549,7 @@ __global__ void k_clip_generic(const int32_t *__restrict__ cand_dst,
     const double *sp = src_cells + 3 * (int64_t)src_nv * s;
     double a = clip_is_dst ? clip_area_buffered(dp, dst_nv, sp, src_nv)
                            : clip_area_buffered(sp, src_nv, dp, dst_nv);
-    double amin = fmin(a_src[s], a_dst[d]);
-    val[c] = a > thr * amin ? a : 0.0;
+    val[c] = kept_area(a, fmin(a_src[s], a_dst[d]), thr);
 }
 
 // ---- wave64 ballot compaction of kept candidates --------------------------
@@ -953,6 +959,15 @@ cr_status cr_regridder_build(cr_regridder **R, const cr_grid *dst, const cr_grid
         auto *r = new cr_regridder();
         try {
             build_regridder(r, dst, src, o, s);
+            for (int side = 0; side < 2; ++side) {
+                const cr_grid *g = side ? src : dst;
+                DevBuf<int32_t> &f = side ? r->fold_src : r->fold_dst;
+                (side ? r->nfold_src : r->nfold_dst) = g->nfold;
+                if (g->nfold) {
+                    f.alloc(2 * g->nfold);
+                    CR_HIP(hipMemcpyAsync(f.p, g->fold.p, f.bytes(), hipMemcpyDeviceToDevice, s));
+                }
+            }
             if (o.normalize) normalize_regridder(r, s);
             r->stats.ms_host =
                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -115,7 +115,7 @@ __global__ void k_area_grad(const double *__restrict__ cells, int nv, const int3
                             const double *__restrict__ gy, int64_t gc, int64_t gf,
                             double *__restrict__ g) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ncells) return;
+    if (t >= ncells || a_to[t] == 0.0) return;  // zero-area (ghost) cells have no area term
     double s = 0.0;
     for (int64_t f = 0; f < nf; ++f) s += gy[t * gc + f * gf] * y[t * yc + f * yf];
     double c = -kappa * s / a_to[t];

@@ -28,6 +28,41 @@ __global__ void k_points_to_cells(const double *__restrict__ pts, int64_t nblock
     }
 }
 
+// Tripolar grid with a RightCenterFolded top row (treeify of
+// ZRegOrthogonalSphericalShellGrid{..., RightCenterFolded},
+// ext/ConservativeRegriddingOceananigansExt.jl:119-187): rows 0..ny-2 are
+// CellBasedGrid cells of the (nx+1) x (ny+1) point matrix; fold-row cells i and
+// nx-1-i are one physical quadrilateral, so only the primaries i < nq and
+// nh <= i < nh+nq (nh = nx/2, nq = nh/2) keep their polygon and the others
+// are zero-area ghosts with every vertex at P[0, ny-1].
+__host__ __device__ inline bool fold_primary(int64_t i, int64_t nh, int64_t nq) {
+    return i < nq || (i >= nh && i < nh + nq);
+}
+
+__global__ void k_tripolar_cells(const double *__restrict__ pts, int64_t nx, int64_t ny,
+                                 double *__restrict__ cells, int32_t *__restrict__ vidx) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nx * ny) return;
+    const int64_t j = c / nx, i = c - j * nx, W = nx + 1;
+    int64_t idx[4] = {i + j * W, i + 1 + j * W, i + 1 + (j + 1) * W, i + (j + 1) * W};
+    if (j == ny - 1 && !fold_primary(i, nx / 2, nx / 4)) idx[0] = idx[1] = idx[2] = idx[3] = (ny - 1) * W;
+    for (int k = 0; k < 4; ++k) {
+        for (int d = 0; d < 3; ++d) cells[(c * 4 + k) * 3 + d] = pts[idx[k] * 3 + d];
+        vidx[c * 4 + k] = (int32_t)idx[k];
+    }
+}
+
+// (primary, partner) cell pairs of the fold row: primary local r -> partner
+// nx-1-r (mirror_fold_partners!, ext/ConservativeRegriddingOceananigansExt.jl:216-240).
+__global__ void k_fold_pairs(int64_t nx, int64_t ny, int32_t *__restrict__ pairs) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nh = nx / 2, nq = nh / 2;
+    if (r >= 2 * nq) return;
+    const int64_t p = r < nq ? r : r + nh - nq, off = (ny - 1) * nx;
+    pairs[2 * r] = (int32_t)(off + p);
+    pairs[2 * r + 1] = (int32_t)(off + nx - 1 - p);
+}
+
 __global__ void k_iota_i32(int32_t *__restrict__ x, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) x[i] = (int32_t)i;
@@ -326,7 +361,7 @@ cr_status cr_grid_from_polygons(cr_grid **grid, const double *xyz, int64_t nvert
             CR_HIP(hipStreamSynchronize(s));  // the host tables end with this scope
             orient_grid(g, s);
         } catch (...) {
-            hipStreamSynchronize(s);
+            (void)hipStreamSynchronize(s);
             delete g;
             throw;
         }
@@ -362,6 +397,50 @@ cr_status cr_grid_from_points(cr_grid **grid, const double *pts, int64_t nblocks
             throw;
         }
         *grid = g;
+    });
+}
+
+cr_status cr_grid_tripolar(cr_grid **grid, const double *pts, int64_t nx, int64_t ny, void *stream) {
+    return guard([&] {
+        CR_REQUIRE(grid && pts && nx >= 2 && ny >= 2, CR_INVALID_ARGUMENT,
+                   "cr_grid_tripolar: need nx >= 2 and ny >= 2");
+        CR_REQUIRE(nx % 2 == 0, CR_INVALID_ARGUMENT, "a RightCenterFolded grid needs an even number of cells in x");
+        CR_REQUIRE(nx * ny < (int64_t(1) << 31), CR_UNSUPPORTED, "more than 2^31-1 cells");
+        check_device();
+        hipStream_t s = as_stream(stream);
+        const int64_t npts = (nx + 1) * (ny + 1);
+        DevBuf<double> dpts((size_t)npts * 3);
+        CR_HIP(hipMemcpyAsync(dpts.p, pts, dpts.bytes(), hipMemcpyHostToDevice, s));
+        auto *g = new cr_grid();
+        try {
+            g->ncells = nx * ny;
+            g->nv = 4;
+            g->convex = 1;
+            g->nverts = npts;
+            g->cells.alloc((size_t)g->ncells * 4 * 3);
+            g->vidx.alloc((size_t)g->ncells * 4);
+            hipLaunchKernelGGL(k_tripolar_cells, dim3(grid_for(g->ncells, 256)), dim3(256), 0, s, dpts.p, nx, ny,
+                               g->cells.p, g->vidx.p);
+            g->nfold = 2 * (nx / 4);
+            if (g->nfold) {
+                g->fold.alloc((size_t)g->nfold * 2);
+                hipLaunchKernelGGL(k_fold_pairs, dim3(grid_for(g->nfold, 256)), dim3(256), 0, s, nx, ny, g->fold.p);
+            }
+            CR_HIP(hipGetLastError());
+            orient_grid(g, s);
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            delete g;
+            throw;
+        }
+        *grid = g;
+    });
+}
+
+cr_status cr_grid_fold_count(const cr_grid *grid, int64_t *npairs) {
+    return guard([&] {
+        CR_REQUIRE(grid && npairs, CR_INVALID_ARGUMENT, "null argument");
+        *npairs = grid->nfold;
     });
 }
 

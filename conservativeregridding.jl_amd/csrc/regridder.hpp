@@ -64,6 +64,10 @@ struct cr_regridder {
     // stored areas = area_scale * unit-sphere areas (radius^2, divided by max(A)
     // after normalize!); 0 when built from arrays (no geometry: no vertex gradients)
     double area_scale = 0.0;
+    // fold partners of the destination / source grid, (primary, partner) pairs:
+    // cr_mirror_fold_partners copies primary -> partner after an apply into that grid
+    cr::DevBuf<int32_t> fold_dst, fold_src;
+    int64_t nfold_dst = 0, nfold_src = 0;
 };
 
 namespace cr {

@@ -249,6 +249,9 @@ class Grid:
         nvert = ctypes.c_int64()
         L.check(L.lib.cr_grid_vertex_count(self._h, ctypes.byref(nvert)))
         self.nverts = nvert.value  # caller-visible vertices (gradients are indexed by these)
+        nf = ctypes.c_int64()
+        L.check(L.lib.cr_grid_fold_count(self._h, ctypes.byref(nf)))
+        self.nfold = nf.value  # (primary, partner) pairs of a folded row
 
     # -- constructors ---------------------------------------------------------
     @classmethod
@@ -318,6 +321,21 @@ class Grid:
         g = cls.from_polygons(*ring_full_polygons(latd, lond), convex=True, stream=stream)
         g.kind = "ringgrid"
         return g
+
+    @classmethod
+    def tripolar(cls, points, stream=None) -> "Grid":
+        """Tripolar grid with a RightCenterFolded top row from its point matrix
+        ``(nx+1, ny+1, 3)`` (the treeify of ext/ConservativeRegriddingOceananigansExt.jl:119-187):
+        fold-row cells i and nx-1-i (0-based) are one physical cell, so only
+        ``i < nx/4`` and ``nx/2 <= i < nx/2 + nx/4`` keep their polygon; the
+        others are zero-area ghosts, filled from their primary after every
+        ``regrid_`` into this grid (``mirror_fold_partners!``)."""
+        c, nb, nx, ny = _point_blocks(points)
+        if nb != 1:
+            raise ArgumentError("a tripolar grid is one point matrix (nx+1, ny+1, 3)")
+        h = ctypes.c_void_p()
+        L.check(L.lib.cr_grid_tripolar(ctypes.byref(h), L.dptr(c), nx, ny, _stream(stream)), "Grid.tripolar")
+        return cls(h.value, (nx, ny), "tripolar")
 
     @classmethod
     def lonlat(cls, ni: int, nj: int, **kw) -> "Grid":

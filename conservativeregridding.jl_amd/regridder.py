@@ -52,6 +52,11 @@ class _Core:
         self.n_dst, self.n_src, self.nnz = n_dst.value, n_src.value, nnz.value
         self._areas = None
         self._csr = {}
+        self.nfold = (0, 0)  # fold pairs of the (destination, source) grid
+
+    def folds(self, transpose: bool) -> int:
+        """Fold pairs of the grid this direction writes into."""
+        return self.nfold[1] if transpose else self.nfold[0]
 
     def areas(self):
         if self._areas is None:
@@ -116,6 +121,7 @@ class Regridder:
         L.check(L.lib.cr_regridder_build(ctypes.byref(h), dg._h, sg._h, ctypes.byref(opts), _stream(stream)),
                 "Regridder")
         self._core, self._t = _Core(h.value), False
+        self._core.nfold = (dg.nfold, sg.nfold)
         self.dst_shape, self.src_shape = dg.shape, sg.shape
 
     @classmethod
@@ -304,12 +310,17 @@ def _merge_fields(shape_s, strides_s, strides_d, dim):
 
 
 def regrid_(dst_field, R: Regridder, src_field, *, dims: int = 1, normalize: bool = True,
-            stream=None):
+            stream=None, mirror_folds: bool = True):
     """``regrid!(dst_field, R, src_field; dims=1, normalize=true)``.
 
     d = (A s) ./ a_d (``normalize=False`` returns A s).  For N-D fields ``dims``
     is the 1-based spatial axis; every other axis is a field index and all
-    fields run in one launch.  Returns ``dst_field``.
+    fields run in one launch.  When the destination grid has a folded row
+    (``Grid.tripolar``), its ghost cells receive their primary's value, as
+    ``finalize_regridding!`` does for Oceananigans fields
+    (ext/ConservativeRegriddingOceananigansExt.jl:207-240); ``mirror_folds=False``
+    leaves them as a plain vector's finalize does (0, or NaN with normalize).
+    Returns ``dst_field``.
     """
     n_out, n_in = R.shape
     src_shape, src_code = _shape_of(src_field)
@@ -349,6 +360,9 @@ def regrid_(dst_field, R: Regridder, src_field, *, dims: int = 1, normalize: boo
         do = sum(i * dst.strides[a] for i, a in zip(idx, outer))
         L.check(L.lib.cr_regrid(R._core.h, transpose_flag, src.dtype_code, nf, src.ptr + so * item, sc, sf,
                                 dst.ptr + do * item, dc, df, 1 if normalize else 0, st), "regrid!")
+        if mirror_folds and R._core.folds(R._t):
+            L.check(L.lib.cr_mirror_fold_partners(R._core.h, transpose_flag, src.dtype_code, nf,
+                                                  dst.ptr + do * item, dc, df, st), "mirror_fold_partners!")
     if dst.writeback is not None:  # finalize_regridding! into a host field
         dst.writeback(dst.keepalive)
     return dst_field
@@ -374,7 +388,8 @@ def regrid(R: Regridder, src_field, **kw):
     return regrid_(out, R, a, **kw)
 
 
-def regrid_plan(dst_field, R: Regridder, src_field, *, dims: int = 1, normalize: bool = True):
+def regrid_plan(dst_field, R: Regridder, src_field, *, dims: int = 1, normalize: bool = True,
+                mirror_folds: bool = True):
     """Validate once and return a zero-argument callable that re-launches
     ``regrid_(dst_field, R, src_field)`` on the current stream with no host work
     beyond the launch (device fields only).  Suitable for timing loops and for
@@ -383,7 +398,7 @@ def regrid_plan(dst_field, R: Regridder, src_field, *, dims: int = 1, normalize:
     if not (isinstance(dst_field, torch.Tensor) and dst_field.is_cuda and isinstance(src_field, torch.Tensor)
             and src_field.is_cuda):
         raise MethodError("regrid_plan needs device (CUDA) tensors")
-    regrid_(dst_field, R, src_field, dims=dims, normalize=normalize)  # validates + first launch
+    regrid_(dst_field, R, src_field, dims=dims, normalize=normalize, mirror_folds=mirror_folds)  # validates
     ns = src_field.ndim
     dim = 0 if ns == 1 else dims - 1
     nf, sf, df, outer = _merge_fields(tuple(src_field.shape), src_field.stride(), dst_field.stride(), dim)
@@ -394,11 +409,18 @@ def regrid_plan(dst_field, R: Regridder, src_field, *, dims: int = 1, normalize:
             dst_field.data_ptr(), dst_field.stride(dim), df, 1 if normalize else 0)
     fn = L.lib.cr_regrid
     keep = (R, dst_field, src_field)
+    mirror = None
+    if mirror_folds and R._core.folds(R._t):
+        margs = (R._core.h, 1 if R._t else 0, code, nf, dst_field.data_ptr(), dst_field.stride(dim), df)
+        mirror = L.lib.cr_mirror_fold_partners
 
     def launch():
-        st = fn(*args, torch.cuda.current_stream().cuda_stream)
+        stream = torch.cuda.current_stream().cuda_stream
+        st = fn(*args, stream)
         if st:
             L.check(st, "regrid!")
+        if mirror is not None:
+            L.check(mirror(*margs, stream), "mirror_fold_partners!")
         return keep[1]
 
     return launch
@@ -438,7 +460,7 @@ def regrid_set_variant(R: Regridder, variant: int, dtype="float32") -> None:
 
 
 def regrid_backward(R: Regridder, dst_grid, src_grid, x, y, gy, *, normalize: bool = True,
-                    field_grad: bool = True, vertex_grad: bool = True):
+                    field_grad: bool = True, vertex_grad: bool = True, mirror_folds: bool = True):
     """Reverse mode of ``y = regrid(R, x)`` (FP64 device tensors, cells along
     dim 0, fields along dim 1 if 2-D).  Returns ``(gx, g_dst, g_src)``: the
     gradient w.r.t. x and w.r.t. the caller vertices of the two grids R was
@@ -449,6 +471,12 @@ def regrid_backward(R: Regridder, dst_grid, src_grid, x, y, gy, *, normalize: bo
         if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64):
             raise MethodError("regrid_backward needs FP64 CUDA tensors")
     nf = 1 if x.ndim == 1 else x.shape[1]
+    if mirror_folds and R._core.folds(R._t):
+        # pullback of the fold mirror: a ghost's gradient belongs to its primary
+        gy = gy.clone(memory_format=torch.contiguous_format)
+        L.check(L.lib.cr_mirror_fold_partners_adjoint(
+            R._core.h, 1 if R._t else 0, nf, gy.data_ptr(), gy.stride(0), gy.stride(1) if gy.ndim > 1 else 1,
+            torch.cuda.current_stream().cuda_stream), "mirror_fold_partners! adjoint")
 
     def st(t):
         return (t.stride(0), t.stride(1) if t.ndim > 1 else 1)

@@ -145,6 +145,17 @@ cr_status cr_grid_from_points(cr_grid **grid, const double *pts, int64_t nblocks
 cr_status cr_grid_from_polygons(cr_grid **grid, const double *xyz, int64_t nverts,
                                 const int64_t *offsets, const int32_t *verts, int64_t ncells,
                                 int32_t convex, void *stream);
+/* Tripolar grid with a RightCenterFolded top row from its (nx+1) x (ny+1)
+ * point matrix (one block, layout of cr_grid_from_points): treeify of
+ * ZRegOrthogonalSphericalShellGrid{..., RightCenterFolded}
+ * (ext/ConservativeRegriddingOceananigansExt.jl:119-187).  Fold-row cells i
+ * and nx-1-i are one physical cell: only i < nx/4 and nx/2 <= i < nx/2+nx/4
+ * keep their polygon, the rest are zero-area ghosts that intersect nothing
+ * (PaddedTreeWrapper, :65-115); (primary, partner) pairs are kept for
+ * cr_mirror_fold_partners.  nx must be even (the reference's assertion). */
+cr_status cr_grid_tripolar(cr_grid **grid, const double *pts, int64_t nx, int64_t ny, void *stream);
+/* Number of (primary, partner) fold pairs of a grid (0 unless folded). */
+cr_status cr_grid_fold_count(const cr_grid *grid, int64_t *npairs);
 /* Generate a HEALPix grid on the device (HealpixRootNode{O}(nside)). */
 cr_status cr_grid_healpix(cr_grid **grid, int64_t nside, int32_t order, int32_t step,
                           void *stream);
@@ -181,6 +192,18 @@ cr_status cr_regridder_normalize(cr_regridder *R, void *stream);
 void cr_regridder_destroy(cr_regridder *R);
 
 /* ------------------------------------------------------------- apply -- */
+/* Fold partners after an apply into a folded grid (the destination grid of
+ * this direction): dst[partner] = dst[primary] for every field, the
+ * finalize_regridding! step for fields on RightCenterFolded grids
+ * (mirror_fold_partners!, ext/ConservativeRegriddingOceananigansExt.jl:207-240).
+ * A no-op for grids without folds.  Layout arguments as cr_regrid. */
+cr_status cr_mirror_fold_partners(const cr_regridder *R, int32_t transpose, int32_t dtype, int64_t nfields,
+                                  void *dst, int64_t dst_cell_stride, int64_t dst_field_stride, void *stream);
+/* Its adjoint on an FP64 gradient, in place: g[primary] += g[partner],
+ * g[partner] = 0 (no reference counterpart: the pullback of the mirror, for
+ * the rrule of regrid into a folded grid). */
+cr_status cr_mirror_fold_partners_adjoint(const cr_regridder *R, int32_t transpose, int64_t nfields, double *g,
+                                          int64_t cell_stride, int64_t field_stride, void *stream);
 /*
  * regrid!(dst, R, src) for nfields fields at once (transpose=1: transpose(R)).
  * src and dst are DEVICE pointers of dtype elements.  Element (cell c, field f)

@@ -304,6 +304,10 @@ static void *build_worker(void *arg) {
                     int64_t kend = (ix * g->nb + iy) * g->nb + iz1;
                     for (int64_t p = lower_bound_key(g->sorted, g->n, kbeg); p < g->n && g->sorted[p].key <= kend; ++p) {
                         int64_t s = g->sorted[p].idx;
+                        /* zero-area cells (padding ghosts) are not in the
+                         * reference's search trees: PaddedTreeWrapper,
+                         * ext/ConservativeRegriddingOceananigansExt.jl:65-115 */
+                        if (c->a_src[s] == 0.0 || c->a_dst[d] == 0.0) continue;
                         cap_t sc = c->src_caps[s];
                         double dx = sc.cx - qc.cx, dy = sc.cy - qc.cy, dz = sc.cz - qc.cz;
                         double rr = sc.r + qc.r;
@@ -392,6 +396,7 @@ void orc_cell_areas(const double *xyz, int64_t ncells, int32_t nv, double *out, 
  * Only rows in [row_begin, row_end) are computed; the rest stay empty (bounded
  * CPU-baseline samples).  Entries with area <= threshold_rel * min(a_src, a_dst)
  * are dropped (threshold_rel = 0 keeps exactly the pairs with area > 0).
+ * A cell of zero area is in no pair.
  */
 orc_result *orc_build(const double *dst_xyz, int64_t n_dst, int32_t dst_nv,
                       const double *src_xyz, int64_t n_src, int32_t src_nv,

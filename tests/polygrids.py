@@ -76,3 +76,37 @@ def rotation(seed):
     if np.linalg.det(q) < 0:
         q[:, 0] = -q[:, 0]
     return q
+
+
+def folded_points(cr, nx, ny, fold_lat):
+    """Point matrix ``(nx+1, ny+1, 3)`` of a grid with a RightCenterFolded top
+    row: vertex rows 0..ny-1 are parallels from -90 to ``fold_lat`` at
+    longitudes i*360/nx, and the top vertex row is row ny-1 reversed
+    (P[k, ny] = P[nx-k, ny-1]), so fold-row cells i and nx-1-i are the same
+    strip of the polar cap between the chords (lon_i, -lon_i) and
+    (lon_i+1, -lon_i+1).  The cells tile the sphere once when each strip is
+    counted once."""
+    lon = np.arange(nx + 1) * (360.0 / nx)
+    lat = -90.0 + np.arange(ny) * ((fold_lat + 90.0) / (ny - 1))
+    L, P = np.meshgrid(lon, lat, indexing="ij")
+    pts = np.empty((nx + 1, ny + 1, 3))
+    pts[:, :ny] = cr.lonlat_to_unit(L, P)
+    pts[:, ny] = pts[::-1, ny - 1]
+    return pts
+
+
+def fold_primary(i, nx):
+    """Fold-row cells that keep their polygon (ext/ConservativeRegriddingOceananigansExt.jl:146-150)."""
+    nh, nq = nx // 2, nx // 4
+    return (i < nq) | ((i >= nh) & (i < nh + nq))
+
+
+def folded_cells(cr, pts):
+    """Oracle input of a folded grid: CellBasedGrid cells, with every non-primary
+    fold-row cell replaced by the degenerate ghost polygon at P[0, ny-1]."""
+    nx, ny = pts.shape[0] - 1, pts.shape[1] - 1
+    cells = cr.points_to_cells(pts)
+    i = np.arange(nx)
+    ghost = (ny - 1) * nx + i[~fold_primary(i, nx)]
+    cells[ghost] = pts[0, ny - 1]
+    return cells, ghost

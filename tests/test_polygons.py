@@ -156,3 +156,37 @@ def test_oracle_regrid_ragged_invariants(cr, orc, clip_is_dst):
     cols = np.bincount(ref.col, weights=ref.val, minlength=ref.n_src)
     np.testing.assert_allclose(cols, ref.a_src, rtol=1.5e-8)
     np.testing.assert_allclose(ref.apply(np.ones(ref.n_src)), 1.0, atol=1e-10)
+
+
+def test_oracle_zero_area_cells_in_no_pair(cr, orc):
+    """A zero-area cell (a padding ghost) intersects nothing, also as the clip
+    polygon, where its zero-length edges would pass the whole subject."""
+    ll = cr.points_to_cells(cr.lonlat_points(12, 6))
+    dst = cr.points_to_cells(cr.lonlat_points(8, 4))
+    dst[5] = dst[5][0]  # all four vertices at one point
+    for clip_is_dst in (True, False):
+        ref = orc.build(dst, ll, clip_is_dst=clip_is_dst)
+        assert ref.a_dst[5] == 0.0
+        assert ref.rowptr[6] == ref.rowptr[5]
+        src = ll.copy()
+        src[7] = src[7][2]
+        ref = orc.build(dst, src, clip_is_dst=clip_is_dst)
+        assert ref.a_src[7] == 0.0 and not np.any(ref.col == 7)
+
+
+def test_folded_grid_partition(cr, orc):
+    """The synthetic RightCenterFolded grid tiles the sphere once: primaries +
+    rest sum to 4 pi, ghosts have zero area, and regridding it to a lon-lat
+    grid reproduces the cell areas as row and column sums."""
+    from polygrids import folded_cells, folded_points
+    pts = folded_points(cr, 24, 10, 60.0)
+    cells, ghost = folded_cells(cr, pts)
+    a = orc.cell_areas(cells)
+    assert np.all(a[ghost] == 0.0) and np.all(np.delete(a, ghost) > 0)
+    assert abs(a.sum() - 4 * math.pi) < 1e-12 * 4 * math.pi
+    ll = cr.points_to_cells(cr.lonlat_points(30, 15))
+    ref = orc.build(ll, cells)
+    cols = np.bincount(ref.col, weights=ref.val, minlength=ref.n_src)
+    np.testing.assert_allclose(cols, ref.a_src, rtol=1.5e-8, atol=0)
+    rows = np.add.reduceat(ref.val, ref.rowptr[:-1])
+    np.testing.assert_allclose(rows, ref.a_dst, rtol=1.5e-8)
