@@ -11,6 +11,10 @@ for the file:line map):
 * :func:`transpose`  -- the CSR of ``transpose(R).intersections`` (regridder.jl:49-50)
 * :func:`clip_area`, :func:`polygon_area`, :func:`cell_areas` -- the per-pair
   operator (regridder.jl:96-103) and ``areas`` (regridder.jl:165-178)
+* :func:`dunavant`, :func:`gll`, :func:`lagrange`, :func:`element_map`,
+  :func:`inverse_element_map`, :func:`element_jacobian`, :func:`se_moments` --
+  the spectral-element weights (cr_oracle_se.c; triangle_quadrature.jl and
+  ext/ConservativeRegriddingClimaCoreExt.jl)
 """
 from __future__ import annotations
 
@@ -71,6 +75,20 @@ def lib():
     L.orc_polygon_area_grad.argtypes = [_dp, ctypes.c_int32, _dp]
     L.orc_clip_area_grad.restype = ctypes.c_double
     L.orc_clip_area_grad.argtypes = [_dp, ctypes.c_int32, _dp, ctypes.c_int32, _dp, _dp]
+    L.orc_dunavant.restype = ctypes.c_int32
+    L.orc_dunavant.argtypes = [ctypes.c_int32, _dp, _dp]
+    L.orc_gll.restype = None
+    L.orc_gll.argtypes = [ctypes.c_int32, _dp, _dp]
+    L.orc_lagrange.restype = None
+    L.orc_lagrange.argtypes = [ctypes.c_int32, _dp, ctypes.c_double, _dp]
+    L.orc_element_map.restype = None
+    L.orc_element_map.argtypes = [_dp, ctypes.c_double, ctypes.c_double, _dp]
+    L.orc_element_jacobian.restype = ctypes.c_double
+    L.orc_element_jacobian.argtypes = [_dp, ctypes.c_double, ctypes.c_double]
+    L.orc_inverse_element_map.restype = ctypes.c_int32
+    L.orc_inverse_element_map.argtypes = [_dp, _dp, _dp, _dp]
+    L.orc_se_moments.restype = ctypes.c_int32
+    L.orc_se_moments.argtypes = [_dp, _dp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp, _dp]
     L.orc_transpose.restype = None
     L.orc_transpose.argtypes = [_i64p, _i32p, _dp, ctypes.c_int64, ctypes.c_int64, _i64p, _i32p, _dp]
     _lib = L
@@ -125,6 +143,62 @@ def clip_polygon(subject, clip) -> np.ndarray:
     if m < 0:
         raise RuntimeError("clip buffer overflow")
     return out[:m].copy()
+
+
+def dunavant(degree: int):
+    """``reference_rule(degree)``: (barycentric points (n, 3), weights (n,))."""
+    bary, w = np.empty((16, 3)), np.empty(16)
+    n = lib().orc_dunavant(degree, _d(bary), _d(w))
+    if n < 0:
+        raise ValueError(f"no triangle rule of degree >= {degree}")
+    return bary[:n].copy(), w[:n].copy()
+
+
+def gll(n: int):
+    """Gauss-Lobatto-Legendre nodes (ascending) and weights, n points."""
+    x, w = np.empty(n), np.empty(n)
+    lib().orc_gll(n, _d(x), _d(w))
+    return x, w
+
+
+def lagrange(nodes, t) -> np.ndarray:
+    x = _f64(nodes)
+    out = np.empty(x.shape[0])
+    lib().orc_lagrange(x.shape[0], _d(x), float(t), _d(out))
+    return out
+
+
+def element_map(corners, xi, eta) -> np.ndarray:
+    c = _f64(corners)
+    out = np.empty(3)
+    lib().orc_element_map(_d(c), float(xi), float(eta), _d(out))
+    return out
+
+
+def element_jacobian(corners, xi, eta) -> float:
+    c = _f64(corners)
+    return lib().orc_element_jacobian(_d(c), float(xi), float(eta))
+
+
+def inverse_element_map(corners, x):
+    c, p = _f64(corners), _f64(x)
+    xi, eta = ctypes.c_double(), ctypes.c_double()
+    if lib().orc_inverse_element_map(_d(c), _d(p), ctypes.byref(xi), ctypes.byref(eta)) != 0:
+        raise ValueError("point outside the element")
+    return xi.value, eta.value
+
+
+def se_moments(element, cell, nq: int, degree: int | None = None) -> np.ndarray:
+    """B[i, j] = integral over (element ∩ cell) of phi_i(xi) phi_j(eta) on the unit
+    sphere (``accumulate_principled_b``), shape (nq, nq) indexed [i, j]."""
+    e, c = _f64(element), _f64(cell)
+    nodes, _ = gll(nq)
+    B = np.empty(nq * nq)
+    m = lib().orc_se_moments(_d(e), _d(c), c.shape[0], nq, 2 * (nq - 1) if degree is None else degree,
+                             _d(nodes), _d(B))
+    if m < 0:
+        raise RuntimeError("se_moments failed (clip overflow or a point outside the element)")
+    return B.reshape(nq, nq).T.copy()
 
 
 def cell_areas(cells, nthreads: int = 0) -> np.ndarray:
