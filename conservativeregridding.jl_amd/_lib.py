@@ -72,6 +72,14 @@ SIGNATURES = {
     "cr_grid_from_points": (c_i32, [ctypes.POINTER(c_vp), _dp, c_i64, c_i64, c_i64, c_vp]),
     "cr_grid_healpix": (c_i32, [ctypes.POINTER(c_vp), c_i64, c_i32, c_i32, c_vp]),
     "cr_grid_tripolar": (c_i32, [ctypes.POINTER(c_vp), _dp, c_i64, c_i64, c_vp]),
+    "cr_se_tri_rule": (c_i32, [c_i32, _dp, _dp, _i32p]),
+    "cr_se_gll": (c_i32, [c_i32, _dp, _dp]),
+    "cr_se_nodes": (c_i32, [c_vp, c_i32, c_dbl, _dp, _dp]),
+    "cr_se_moments": (c_i32, [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp]),
+    "cr_se_regridder_to_fv": (c_i32, [ctypes.POINTER(c_vp), c_vp, c_vp, c_i32, c_i32, _dp,
+                                      ctypes.POINTER(BuildOpts), c_vp]),
+    "cr_se_regridder_from_fv": (c_i32, [ctypes.POINTER(c_vp), c_vp, c_vp, c_i32, c_i32, _dp,
+                                        ctypes.POINTER(BuildOpts), c_vp]),
     "cr_grid_fold_count": (c_i32, [c_vp, _i64p]),
     "cr_mirror_fold_partners": (c_i32, [c_vp, c_i32, c_i32, c_i64, c_vp, c_i64, c_i64, c_vp]),
     "cr_mirror_fold_partners_adjoint": (c_i32, [c_vp, c_i32, c_i64, c_vp, c_i64, c_i64, c_vp]),

@@ -717,6 +717,32 @@ int bits_for(int64_t n) {
     return b;
 }
 
+// CSR(A^T) of a device CSR(A) whose entry k lies in row coo_row[k]: a stable
+// radix sort by column keeps the rows of A ascending inside each row of A^T.
+void csr_transpose(const Csr &F, const int32_t *coo_row, Csr &Bk, hipStream_t s) {
+    const int64_t nnz = F.nnz, ns = F.ncols;
+    const int B = 256;
+    Bk.nrows = ns; Bk.ncols = F.nrows; Bk.nnz = nnz;
+    Bk.rowptr.alloc_kept(ns + 1, s);
+    Bk.col.alloc_kept(nnz, s);
+    Bk.val.alloc_kept(nnz, s);
+    if (nnz == 0) {
+        CR_HIP(hipMemsetAsync(Bk.rowptr.p, 0, Bk.rowptr.bytes(), s));
+        return;
+    }
+    Temp T(s);
+    DevBuf<int32_t> iota(nnz, s), perm_t(nnz, s), tkey(nnz, s);
+    hipLaunchKernelGGL(k_iota, dim3(grid_for(nnz, B)), dim3(B), 0, s, iota.p, nnz);
+    sort_pairs(T, F.col.p, tkey.p, iota.p, perm_t.p, nnz, bits_for(ns), s);
+    hipLaunchKernelGGL(k_gather<int32_t>, dim3(grid_for(nnz, B)), dim3(B), 0, s, coo_row, perm_t.p, nnz,
+                       Bk.col.p);
+    hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(nnz, B)), dim3(B), 0, s, F.val.p, perm_t.p, nnz,
+                       Bk.val.p);
+    hipLaunchKernelGGL(k_rowptr_from_sorted, dim3(grid_for(ns + 1, B)), dim3(B), 0, s, tkey.p, nnz, ns,
+                       Bk.rowptr.p);
+    CR_HIP(hipGetLastError());
+}
+
 void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
                      const cr_build_opts &o, hipStream_t s) {
     const int64_t nd = dst->ncells, ns = src->ncells;
@@ -834,15 +860,10 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     CR_HIP(hipStreamSynchronize(s));
 
     Csr &F = R->fwd;
-    Csr &Bk = R->bwd;
     F.nrows = nd; F.ncols = ns; F.nnz = nnz;
-    Bk.nrows = ns; Bk.ncols = nd; Bk.nnz = nnz;
     F.rowptr.alloc_kept(nd + 1, s);
-    Bk.rowptr.alloc_kept(ns + 1, s);
     F.col.alloc_kept(nnz, s);
     F.val.alloc_kept(nnz, s);
-    Bk.col.alloc_kept(nnz, s);
-    Bk.val.alloc_kept(nnz, s);
     {
         // CSR(A): kept pairs in candidate order (columns in search order within a row)
         DevBuf<int32_t> coo_row(nnz, s);
@@ -852,21 +873,7 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
         hipLaunchKernelGGL(k_rowptr_from_mask, dim3(grid_for(nd + 1, B)), dim3(B), 0, s, cand_off.p, nd, ncand,
                            blk.p, keep_mask.p, F.rowptr.p);
         CR_HIP(hipGetLastError());
-        // CSR(A^T): stable sort by column keeps the rows of A ascending inside each row of A^T
-        if (nnz) {
-            DevBuf<int32_t> iota(nnz, s), perm_t(nnz, s), tkey(nnz, s);
-            hipLaunchKernelGGL(k_iota, dim3(grid_for(nnz, B)), dim3(B), 0, s, iota.p, nnz);
-            sort_pairs(T, F.col.p, tkey.p, iota.p, perm_t.p, nnz, bits_for(ns), s);
-            hipLaunchKernelGGL(k_gather<int32_t>, dim3(grid_for(nnz, B)), dim3(B), 0, s, coo_row.p, perm_t.p,
-                               nnz, Bk.col.p);
-            hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(nnz, B)), dim3(B), 0, s, F.val.p, perm_t.p,
-                               nnz, Bk.val.p);
-            hipLaunchKernelGGL(k_rowptr_from_sorted, dim3(grid_for(ns + 1, B)), dim3(B), 0, s, tkey.p, nnz, ns,
-                               Bk.rowptr.p);
-            CR_HIP(hipGetLastError());
-        } else {
-            CR_HIP(hipMemsetAsync(Bk.rowptr.p, 0, Bk.rowptr.bytes(), s));
-        }
+        csr_transpose(F, coo_row.p, R->bwd, s);
     }
 
     // 6. radius^2 scaling, areas
@@ -875,7 +882,7 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     R->a_src = std::move(a_src);
     if (r2 != 1.0) {
         hipLaunchKernelGGL(k_scale, dim3(grid_for(nnz, B)), dim3(B), 0, s, F.val.p, nnz, r2, 1.0);
-        hipLaunchKernelGGL(k_scale, dim3(grid_for(nnz, B)), dim3(B), 0, s, Bk.val.p, nnz, r2, 1.0);
+        hipLaunchKernelGGL(k_scale, dim3(grid_for(nnz, B)), dim3(B), 0, s, R->bwd.val.p, nnz, r2, 1.0);
         hipLaunchKernelGGL(k_scale, dim3(grid_for(nd, B)), dim3(B), 0, s, R->a_dst.p, nd, r2, 1.0);
         hipLaunchKernelGGL(k_scale, dim3(grid_for(ns, B)), dim3(B), 0, s, R->a_src.p, ns, r2, 1.0);
         CR_HIP(hipGetLastError());

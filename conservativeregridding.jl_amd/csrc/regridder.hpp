@@ -74,4 +74,9 @@ namespace cr {
 // Finish a regridder whose CSR arrays and areas are filled: inverse areas and
 // apply plans.
 void finalize_regridder(cr_regridder *R, hipStream_t s);
+// The device build of Regridder(dst, src) into R (no normalize!).
+void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src, const cr_build_opts &o,
+                     hipStream_t s);
+// CSR(A^T) of the device CSR(A) F whose entry k lies in row coo_row[k].
+void csr_transpose(const Csr &F, const int32_t *coo_row, Csr &Bk, hipStream_t s);
 }  // namespace cr

@@ -99,24 +99,33 @@ class Regridder:
       dropped (default 1e-14; 0 keeps every positive area like the reference)
     * ``radius`` -- sphere radius (areas in radius^2 units)
     * ``clip_side`` -- "auto", "dst" or "src": which side's (convex) cells clip
+    * ``triangle_quad_degree`` -- Dunavant rule of the spectral-element weights
+      when one side is a :class:`SpectralElementSpace` (default 2 (nq - 1))
     """
 
     def __init__(self, dst, src, *, normalize: bool = False, intersection_operator=None,
                  threaded=True, threshold_rel: float = 1e-14, radius: float = EARTH_RADIUS,
-                 clip_side: str = "auto", stream=None, _core=None, _transposed=False):
+                 clip_side: str = "auto", triangle_quad_degree=None, stream=None, _core=None,
+                 _transposed=False):
         if _core is not None:
             self._core, self._t = _core, _transposed
             return
         if intersection_operator is not None:
             raise MethodError("a custom intersection_operator runs per pair on the CPU; the device build "
                               "computes spherical clip areas only")
-        dg, sg = as_grid(dst), as_grid(src)
         opts = L.BuildOpts()
         L.lib.cr_build_opts_default(ctypes.byref(opts))
         opts.threshold_rel = float(threshold_rel)
         opts.radius = float(radius)
         opts.normalize = 1 if normalize else 0
         opts.clip_side = {"auto": 0, "dst": 1, "src": 2}[clip_side]
+        from .se import SpectralElementSpace, build_se_regridder
+        if isinstance(dst, SpectralElementSpace) or isinstance(src, SpectralElementSpace):
+            # the ClimaCore extension's constructors (ext/ConservativeRegriddingClimaCoreExt.jl:416-426, 489-508)
+            h, self.dst_shape, self.src_shape = build_se_regridder(dst, src, opts, triangle_quad_degree, stream)
+            self._core, self._t = _Core(h), False
+            return
+        dg, sg = as_grid(dst), as_grid(src)
         h = ctypes.c_void_p()
         L.check(L.lib.cr_regridder_build(ctypes.byref(h), dg._h, sg._h, ctypes.byref(opts), _stream(stream)),
                 "Regridder")

@@ -300,6 +300,50 @@ cr_status cr_esmf_write(const char *path, int64_t n_s, const double *S, const in
                         int64_t n_b, const double *frac_b, const double *area_b,
                         const cr_esmf_attrs *attrs);
 
+/* ---------------------------------------------- spectral elements -- */
+/* The ClimaCore extension's regridders (ext/ConservativeRegriddingClimaCoreExt.jl)
+ * for spectral-element spaces given by their elements: a grid of
+ * quadrilaterals whose vertices c1..c4 (ring order, counter-clockwise) are the
+ * corners (-1,-1), (1,-1), (1,1), (-1,1) of the NormalizedBilinearMap
+ * x(xi, eta) = p / |p|, p the bilinear interpolation of the corners, and nq
+ * Gauss-Lobatto-Legendre nodes per direction (2 <= nq <= 8).  Nodes are
+ * numbered e*nq^2 + i + j*nq (element-major, i fastest: se_node_positions,
+ * :136-145).  tri_degree 0 selects 2*(nq-1) (the reference's default). */
+
+/* reference_rule(degree) (src/regridder/triangle_quadrature.jl:139-156): the
+ * Dunavant rule of that degree or the next one up (max 8); bary 3*16 and w 16
+ * doubles. */
+cr_status cr_se_tri_rule(int32_t degree, double *bary, double *w, int32_t *npts);
+/* Gauss-Lobatto-Legendre nodes (ascending) and weights, 2 <= n <= 64. */
+cr_status cr_se_gll(int32_t n, double *x, double *w);
+/* Node positions (3 per node) and weights W = w_i w_j J(xi_i, eta_j) radius^2
+ * (se_node_positions / se_node_weights, :136-158), host outputs, either may be
+ * NULL. */
+cr_status cr_se_nodes(const cr_grid *elements, int32_t nq, double radius, double *xyz, double *weights);
+/* accumulate_principled_b (:248-298) for every non-zero of R (built from
+ * cells as destination and elements as source): B[k*nq^2 + i + j*nq] = the
+ * integral over (element ∩ cell) of phi_i(xi) phi_j(eta), in R's area units;
+ * B is a DEVICE array of nnz*nq^2 doubles in R's device CSR order, whose
+ * (cell, element) pairs go to the optional DEVICE arrays pair_cell and
+ * pair_elem (nnz int32 each). */
+cr_status cr_se_moments(const cr_regridder *R, const cr_grid *cells, const cr_grid *elements, int32_t nq,
+                        int32_t tri_degree, double *B, int32_t *pair_cell, int32_t *pair_elem, void *stream);
+/* Regridder(cells, se_space): se_to_fv_principled (:424-478).  Rows = cells,
+ * columns = nodes, values = the moments; dst areas = cell areas, src areas =
+ * node weights (node_weights, host, nelements*nq^2, or NULL for the map's
+ * own).  opts: radius and threshold_rel of the pair search (normalize ignored). */
+cr_status cr_se_regridder_to_fv(cr_regridder **R, const cr_grid *cells, const cr_grid *elements, int32_t nq,
+                                int32_t tri_degree, const double *node_weights, const cr_build_opts *opts,
+                                void *stream);
+/* Regridder(se_space, cells): fv_to_se_l2_projection (:562-676).  Rows =
+ * nodes, columns = cells; per element the covered part of the local mass
+ * matrix (rows rescaled to the moments' sums) is inverted into the weights;
+ * dst areas = 1, src areas = cell areas.  The per-element solves run on host
+ * threads. */
+cr_status cr_se_regridder_from_fv(cr_regridder **R, const cr_grid *elements, const cr_grid *cells, int32_t nq,
+                                  int32_t tri_degree, const double *node_weights, const cr_build_opts *opts,
+                                  void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,6 +13,9 @@
  *   accumulate_principled_b           ext/ConservativeRegriddingClimaCoreExt.jl:248-298
  *                                     (clip element by cell, fan-triangulate the
  *                                      intersection from its normalised vertex mean,
+ *                                      after merging repeated vertices: a clip repeats
+ *                                      a vertex where a cell corner lies on a clip
+ *                                      edge, as many times as rounding decides,
  *                                      per triangle: area A_t (skip if <= 0), points
  *                                      normalize(l1 c + l2 v1 + l3 v2), weight 2 A_t w,
  *                                      B[i,j] += w phi_i(xi) phi_j(eta))
@@ -289,6 +292,19 @@ int32_t orc_se_moments(const double *elem, const double *cell, int32_t nc, int32
     double bary[48], w[16], phx[16], phy[16];
     memset(B, 0, sizeof(double) * nq * nq);
     int m = orc_clip_polygon(elem, 4, cell, nc, poly, 64);
+    if (m > 1) { /* merge cyclically consecutive vertices closer than 1e-13 (chord) */
+        int wv = 1;
+        for (int v = 1; v < m; ++v) {
+            sv3 d = s_sub(s_load(poly + 3 * v), s_load(poly + 3 * (wv - 1)));
+            if (sqrt(s_dot(d, d)) >= 1e-13) { memmove(poly + 3 * wv, poly + 3 * v, 3 * sizeof(double)); ++wv; }
+        }
+        while (wv > 1) {
+            sv3 d = s_sub(s_load(poly + 3 * (wv - 1)), s_load(poly));
+            if (sqrt(s_dot(d, d)) >= 1e-13) break;
+            --wv;
+        }
+        m = wv;
+    }
     if (m < 3) return m;
     int npts = orc_dunavant(degree, bary, w);
     if (npts < 0) return -1;

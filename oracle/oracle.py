@@ -296,3 +296,95 @@ def transpose(rowptr, col, val, nrows: int, ncols: int):
     lib().orc_transpose(rowptr.ctypes.data_as(_i64p), col.ctypes.data_as(_i32p), _d(val), nrows, ncols,
                         trp.ctypes.data_as(_i64p), tcol.ctypes.data_as(_i32p), _d(tval))
     return trp, tcol, tval
+
+
+# ------------------------------------------------------ spectral elements --
+def se_node_weights(elements, nq: int) -> np.ndarray:
+    """W = w_i w_j J(xi_i, eta_j) on the unit sphere, node order e*nq^2 + i + j*nq."""
+    x, w = gll(nq)
+    E = _f64(elements)
+    out = np.empty((E.shape[0], nq, nq))  # [e, j, i]
+    for e in range(E.shape[0]):
+        for j in range(nq):
+            for i in range(nq):
+                out[e, j, i] = w[i] * w[j] * element_jacobian(E[e], x[i], x[j])
+    return out.reshape(-1)
+
+
+def se_to_fv(cells, elements, nq: int, degree: int | None = None, node_weights=None) -> OracleCSR:
+    """se_to_fv_principled (ext/ConservativeRegriddingClimaCoreExt.jl:424-478) on
+    the unit sphere: rows = cells, columns = nodes e*nq^2 + i + j*nq, values the
+    moments (exact zeros dropped); dst areas = cell areas, src areas = W."""
+    C, E = _f64(cells), _f64(elements)
+    P = build(C, E)
+    Q = nq * nq
+    rows, cols, vals = [], [], []
+    for c in range(P.n_dst):
+        for k in range(P.rowptr[c], P.rowptr[c + 1]):
+            e = int(P.col[k])
+            B = se_moments(E[e], C[c], nq, degree).flatten(order="F")  # index i + j*nq
+            nz = np.nonzero(B)[0]
+            rows.append(np.full(nz.size, c))
+            cols.append(e * Q + nz)
+            vals.append(B[nz])
+    r, cc, v = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+    order = np.lexsort((cc, r))
+    r, cc, v = r[order], cc[order], v[order]
+    rp = np.zeros(P.n_dst + 1, np.int64)
+    np.add.at(rp, r + 1, 1)
+    rp = np.cumsum(rp)
+    W = se_node_weights(E, nq) if node_weights is None else _f64(node_weights)
+    return OracleCSR(P.n_dst, E.shape[0] * Q, rp, cc.astype(np.int32), v, P.a_dst, W, P.n_candidates, 0.0, 1)
+
+
+def fv_to_se(elements, cells, nq: int, degree: int | None = None, node_weights=None) -> OracleCSR:
+    """fv_to_se_l2_projection (ext/ConservativeRegriddingClimaCoreExt.jl:562-676) on
+    the unit sphere: per element, the local mass matrix (Jacobian interpolated
+    from W / (w_i w_j), GLL rule of ceil((3nq-2)/2)+1 points), its rows
+    rescaled to the element's summed moments, restricted to the covered nodes,
+    inverted and applied to each cell's moments; dst areas 1, src areas = cell areas."""
+    C, E = _f64(cells), _f64(elements)
+    P = build(C, E)
+    T = P.transpose()  # rows = elements, cells ascending
+    Q = nq * nq
+    x, w = gll(nq)
+    W = (se_node_weights(E, nq) if node_weights is None else _f64(node_weights)).reshape(-1, nq, nq)  # [e, j, i]
+    nm = -(-(3 * nq - 2) // 2) + 1
+    xm, wm = gll(nm)
+    ph = np.array([lagrange(x, t) for t in xm])  # [p, a]
+    rows, cols, vals = [], [], []
+    for e in range(E.shape[0]):
+        cl = T.col[T.rowptr[e]:T.rowptr[e + 1]]
+        if cl.size == 0:
+            continue
+        Bs = [se_moments(E[e], C[c], nq, degree).flatten(order="F") for c in cl]
+        bfull = np.sum(Bs, axis=0)
+        cov = np.nonzero(bfull)[0]
+        if cov.size == 0:
+            continue
+        Jn = W[e] / np.outer(w, w)  # [b, a] = W[e, j=b, i=a] / (w_a w_b)
+        M = np.zeros((Q, Q))
+        for q in range(nm):
+            for p in range(nm):
+                J = ph[p] @ Jn.T @ ph[q]  # sum_ab Jn[b, a] phi_a(x_p) phi_b(x_q)
+                f = np.outer(ph[q], ph[p]).reshape(-1)  # index a + b*nq: phi_a(x_p) phi_b(x_q)
+                M += wm[p] * wm[q] * J * np.outer(f, f)
+        Mc = M[np.ix_(cov, cov)]
+        rs = Mc.sum(axis=1)
+        scale = np.where(rs != 0, bfull[cov] / np.where(rs != 0, rs, 1.0), 1.0)
+        Mc = Mc * scale[:, None]
+        Mi = np.linalg.inv(Mc)
+        for c, b in zip(cl, Bs):
+            new = Mi @ b[cov]
+            nz = new != 0
+            rows.append(e * Q + cov[nz])
+            cols.append(np.full(int(nz.sum()), c))
+            vals.append(new[nz])
+    r, cc, v = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+    order = np.lexsort((cc, r))
+    r, cc, v = r[order], cc[order], v[order]
+    n = E.shape[0] * Q
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, r + 1, 1)
+    rp = np.cumsum(rp)
+    return OracleCSR(n, C.shape[0], rp, cc.astype(np.int32), v, np.ones(n), P.a_dst, P.n_candidates, 0.0, 1)
