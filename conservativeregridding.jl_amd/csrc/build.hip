@@ -45,6 +45,39 @@ __global__ void k_cell_bounds(const double *__restrict__ cells, int64_t n, int n
     cap[c] = cell_cap(p, nv);
 }
 
+// The same for cells of NV vertices, 64 per block: the block's cells are one
+// contiguous run of 64*NV*3 doubles, read with coalesced 16-byte loads into LDS
+// (a thread-per-cell read strides 24*NV bytes between lanes), then every lane
+// works on its own cell from LDS.
+template <int NV>
+__global__ void __launch_bounds__(64) k_cell_bounds_staged(const double *__restrict__ cells, int64_t n,
+                                                           double *__restrict__ area, float4 *__restrict__ cap) {
+    constexpr int kPer = 3 * NV;
+    __shared__ double2 buf[64 * kPer / 2];
+    const int64_t c0 = (int64_t)blockIdx.x * 64;
+    const int64_t cnt = min((int64_t)64, n - c0);
+    const double2 *src = reinterpret_cast<const double2 *>(cells + kPer * c0);
+    const int nload = (int)(cnt * kPer / 2);  // kPer*cnt is even for even NV
+    for (int t = threadIdx.x; t < nload; t += 64) buf[t] = src[t];
+    __syncthreads();
+    if (threadIdx.x >= cnt) return;
+    const double *p = reinterpret_cast<const double *>(buf) + kPer * threadIdx.x;
+    area[c0 + threadIdx.x] = ring_area(p, NV);
+    cap[c0 + threadIdx.x] = cell_cap(p, NV);
+}
+
+void launch_cell_bounds(const cr_grid *g, double *area, float4 *cap, hipStream_t s) {
+    const int64_t n = g->ncells;
+    const unsigned blocks = grid_for(n, 64);
+    switch (g->nv) {
+    case 4: hipLaunchKernelGGL(k_cell_bounds_staged<4>, dim3(blocks), dim3(64), 0, s, g->cells.p, n, area, cap); break;
+    case 8: hipLaunchKernelGGL(k_cell_bounds_staged<8>, dim3(blocks), dim3(64), 0, s, g->cells.p, n, area, cap); break;
+    case 16: hipLaunchKernelGGL(k_cell_bounds_staged<16>, dim3(blocks), dim3(64), 0, s, g->cells.p, n, area, cap); break;
+    default:
+        hipLaunchKernelGGL(k_cell_bounds, dim3(grid_for(n, 256)), dim3(256), 0, s, g->cells.p, n, g->nv, area, cap);
+    }
+}
+
 __global__ void k_sfc_keys(const float4 *__restrict__ cap, int64_t n, uint32_t *__restrict__ key,
                          int32_t *__restrict__ idx) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -682,17 +715,6 @@ struct Timer {
     }
 };
 
-// hipcub temporary storage, grown on demand.
-struct Temp {
-    hipStream_t s;
-    DevBuf<unsigned char> buf;
-    explicit Temp(hipStream_t st) : s(st) {}
-    void *get(size_t bytes) {
-        if (buf.n < bytes) buf.alloc_async(std::max(bytes, (size_t)256), s);
-        return buf.p;
-    }
-};
-
 template <class In, class Out>
 void exclusive_sum(Temp &T, In in, Out out, int64_t n, hipStream_t s) {
     size_t bytes = 0;
@@ -719,7 +741,7 @@ int bits_for(int64_t n) {
 
 // CSR(A^T) of a device CSR(A) whose entry k lies in row coo_row[k]: a stable
 // radix sort by column keeps the rows of A ascending inside each row of A^T.
-void csr_transpose(const Csr &F, const int32_t *coo_row, Csr &Bk, hipStream_t s) {
+void csr_transpose(const Csr &F, const int32_t *coo_row, Csr &Bk, Temp &T, hipStream_t s) {
     const int64_t nnz = F.nnz, ns = F.ncols;
     const int B = 256;
     Bk.nrows = ns; Bk.ncols = F.nrows; Bk.nnz = nnz;
@@ -730,7 +752,6 @@ void csr_transpose(const Csr &F, const int32_t *coo_row, Csr &Bk, hipStream_t s)
         CR_HIP(hipMemsetAsync(Bk.rowptr.p, 0, Bk.rowptr.bytes(), s));
         return;
     }
-    Temp T(s);
     DevBuf<int32_t> iota(nnz, s), perm_t(nnz, s), tkey(nnz, s);
     hipLaunchKernelGGL(k_iota, dim3(grid_for(nnz, B)), dim3(B), 0, s, iota.p, nnz);
     sort_pairs(T, F.col.p, tkey.p, iota.p, perm_t.p, nnz, bits_for(ns), s);
@@ -767,10 +788,8 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     a_dst.alloc_kept(nd, s);
     a_src.alloc_kept(ns, s);
     DevBuf<float4> cap_dst(nd, s), cap_src(ns, s);
-    hipLaunchKernelGGL(k_cell_bounds, dim3(grid_for(nd, B)), dim3(B), 0, s, dst->cells.p, nd, dst->nv,
-                       a_dst.p, cap_dst.p);
-    hipLaunchKernelGGL(k_cell_bounds, dim3(grid_for(ns, B)), dim3(B), 0, s, src->cells.p, ns, src->nv,
-                       a_src.p, cap_src.p);
+    launch_cell_bounds(dst, a_dst.p, cap_dst.p, s);
+    launch_cell_bounds(src, a_src.p, cap_src.p, s);
     CR_HIP(hipGetLastError());
     tm.mark();  // 1
 
@@ -873,7 +892,7 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
         hipLaunchKernelGGL(k_rowptr_from_mask, dim3(grid_for(nd + 1, B)), dim3(B), 0, s, cand_off.p, nd, ncand,
                            blk.p, keep_mask.p, F.rowptr.p);
         CR_HIP(hipGetLastError());
-        csr_transpose(F, coo_row.p, R->bwd, s);
+        csr_transpose(F, coo_row.p, R->bwd, T, s);
     }
 
     // 6. radius^2 scaling, areas

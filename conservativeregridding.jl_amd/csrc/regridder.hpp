@@ -77,6 +77,16 @@ void finalize_regridder(cr_regridder *R, hipStream_t s);
 // The device build of Regridder(dst, src) into R (no normalize!).
 void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src, const cr_build_opts &o,
                      hipStream_t s);
+// hipcub temporary storage, grown on demand.
+struct Temp {
+    hipStream_t s;
+    DevBuf<unsigned char> buf;
+    explicit Temp(hipStream_t st) : s(st) {}
+    void *get(size_t bytes) {
+        if (buf.n < bytes) buf.alloc_async(bytes > 256 ? bytes : (size_t)256, s);
+        return buf.p;
+    }
+};
 // CSR(A^T) of the device CSR(A) F whose entry k lies in row coo_row[k].
-void csr_transpose(const Csr &F, const int32_t *coo_row, Csr &Bk, hipStream_t s);
+void csr_transpose(const Csr &F, const int32_t *coo_row, Csr &Bk, Temp &T, hipStream_t s);
 }  // namespace cr

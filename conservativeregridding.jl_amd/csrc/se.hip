@@ -596,7 +596,8 @@ cr_status cr_se_regridder_to_fv(cr_regridder **out, const cr_grid *cells, const 
             hipLaunchKernelGGL(k_scale_rowptr, dim3(grid_for(F.nrows + 1, 256)), dim3(256), 0, s, P->fwd.rowptr.p,
                                F.nrows, (int64_t)Q, F.rowptr.p);
             CR_HIP(hipGetLastError());
-            csr_transpose(F, orow.p, R->bwd, s);
+            Temp T(s);
+            csr_transpose(F, orow.p, R->bwd, T, s);
         }
         R->a_dst = std::move(P->a_dst);
         node_weights(elems, node_set(nq), P->area_scale, node_weights_host, R->a_src, s);

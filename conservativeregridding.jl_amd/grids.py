@@ -241,6 +241,7 @@ class Grid:
 
     def __init__(self, handle, shape, kind: str):
         self._h = ctypes.c_void_p(handle)
+        self._destroy = L.lib.cr_grid_destroy  # kept: module globals may be gone at interpreter exit
         n, nv, cv = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
         L.check(L.lib.cr_grid_info(self._h, ctypes.byref(n), ctypes.byref(nv), ctypes.byref(cv)))
         self.ncells, self.nv, self.convex = n.value, nv.value, bool(cv.value)
@@ -369,9 +370,9 @@ class Grid:
         return self.ncells
 
     def __del__(self):
-        h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            L.lib.cr_grid_destroy(h)
+        h, destroy = getattr(self, "_h", None), getattr(self, "_destroy", None)
+        if h is not None and h.value and destroy is not None:
+            destroy(h)
             self._h = ctypes.c_void_p()
 
     def __repr__(self):

@@ -47,6 +47,7 @@ class _Core:
 
     def __init__(self, handle):
         self.h = ctypes.c_void_p(handle)
+        self._destroy = L.lib.cr_regridder_destroy  # kept: module globals may be gone at interpreter exit
         n_dst, n_src, nnz = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         L.check(L.lib.cr_regridder_info(self.h, ctypes.byref(n_dst), ctypes.byref(n_src), ctypes.byref(nnz)))
         self.n_dst, self.n_src, self.nnz = n_dst.value, n_src.value, nnz.value
@@ -82,9 +83,9 @@ class _Core:
         self._csr = {}
 
     def __del__(self):
-        h = getattr(self, "h", None)
-        if h is not None and h.value:
-            L.lib.cr_regridder_destroy(h)
+        h, destroy = getattr(self, "h", None), getattr(self, "_destroy", None)
+        if h is not None and h.value and destroy is not None:
+            destroy(h)
             self.h = ctypes.c_void_p()
 
 
