@@ -426,12 +426,13 @@ __global__ void __launch_bounds__(64) k_search(const float4 *__restrict__ qcap,
 
 // Candidates whose subject lies outside one clip edge (about a quarter on the
 // benchmark grids) or entirely inside the clip polygon (area = the subject's
-// area) are settled by a cheap uniform test; the others are queued in LDS and
-// clipped 64 at a time, so every lane of the wave runs the expensive pipeline
-// on a real pair.  One wave per kClipChunks * 64
-// candidates; results go to each candidate's own slot.  (Used with a positive
-// drop threshold: a pair the test settles could at most leave a sub-rounding
-// sliver in the pipeline, which the threshold drops too.)
+// area) are settled by a cheap uniform test; the others are queued in LDS by
+// the number of clip edges their subject crosses and clipped 64 at a time by a
+// pipeline with one stage per crossed edge, so every lane of the wave runs the
+// same pipeline on a real pair.  One wave per kClipChunks * 64 candidates;
+// results go to each candidate's own slot.  (Used with a positive drop
+// threshold: a pair the test settles could at most leave a sub-rounding sliver
+// in the pipeline, which the threshold drops too.)
 constexpr int kClipChunks = 8;
 
 // The kept value of a pair: its area if above threshold_rel times the smaller
@@ -443,21 +444,56 @@ __device__ __forceinline__ double kept_area(double a, double amin, double thr) {
     return amin > 0.0 && a > thr * amin ? a : 0.0;
 }
 
-template <int NC>
-__device__ __forceinline__ void clip_candidate(int64_t c, const int32_t *__restrict__ cand_dst,
-                                               const int32_t *__restrict__ cand_src,
-                                               const double *__restrict__ dst_cells, int dst_nv,
-                                               const double *__restrict__ src_cells, int src_nv,
-                                               int clip_is_dst, const double *__restrict__ a_dst,
-                                               const double *__restrict__ a_src, double thr,
-                                               double *__restrict__ val) {
+template <int NS, int NC>
+__device__ __forceinline__ void clip_candidate_masked(int64_t c, unsigned mask, const int32_t *__restrict__ cand_dst,
+                                                      const int32_t *__restrict__ cand_src,
+                                                      const double *__restrict__ dst_cells, int dst_nv,
+                                                      const double *__restrict__ src_cells, int src_nv,
+                                                      int clip_is_dst, const double *__restrict__ a_dst,
+                                                      const double *__restrict__ a_src, double thr,
+                                                      double *__restrict__ val) {
     const int64_t d = cand_dst[c], s = cand_src[c];
     const double *dp = dst_cells + 3 * (int64_t)dst_nv * d;
     const double *sp = src_cells + 3 * (int64_t)src_nv * s;
     const double *clip = clip_is_dst ? dp : sp, *subj = clip_is_dst ? sp : dp;
     const int nc = clip_is_dst ? dst_nv : src_nv, ns = clip_is_dst ? src_nv : dst_nv;
-    const double a = clip_area_stream<NC>(clip, nc, subj, ns);
+    const double a = clip_area_stream_masked<NS, NC>(clip, nc, mask, subj, ns);
     val[c] = kept_area(a, fmin(a_src[s], a_dst[d]), thr);
+}
+
+// Clip the queued candidates 64 at a time with an NS-stage pipeline (entries:
+// local index | crossed-edge mask << 16; a mask of fewer bits gets pass-through
+// stages).  Full batches only, unless `drain`.  Block = one wave, so qn is
+// uniform.
+template <int NS, int NC>
+__device__ __forceinline__ void clip_queued(int32_t *q, int &qn, bool drain, int64_t base,
+                                            const int32_t *__restrict__ cand_dst, const int32_t *__restrict__ cand_src,
+                                            const double *__restrict__ dst_cells, int dst_nv,
+                                            const double *__restrict__ src_cells, int src_nv, int clip_is_dst,
+                                            const double *__restrict__ a_dst, const double *__restrict__ a_src,
+                                            double thr, double *__restrict__ val) {
+    const int lane = threadIdx.x;
+    while (qn >= 64 || (drain && qn > 0)) {
+        if (lane < qn) {
+            const int32_t e = q[lane];
+            clip_candidate_masked<NS, NC>(base + (e & 0xffff), pad_mask((unsigned)e >> 16, NS), cand_dst, cand_src,
+                                          dst_cells, dst_nv, src_cells, src_nv, clip_is_dst, a_dst, a_src, thr, val);
+        }
+        const int left = qn > 64 ? qn - 64 : 0;
+        const int moved = lane < left ? q[64 + lane] : 0;
+        __syncthreads();
+        if (lane < left) q[lane] = moved;
+        qn = left;
+        __syncthreads();
+    }
+}
+
+// Move a queue's entries (fewer than 64) to the end of a longer-stage queue.
+__device__ __forceinline__ void promote(const int32_t *from, int &nfrom, int32_t *to, int &nto) {
+    if ((int)threadIdx.x < nfrom) to[nto + threadIdx.x] = from[threadIdx.x];
+    nto += nfrom;
+    nfrom = 0;
+    __syncthreads();
 }
 
 template <int NC>
@@ -468,21 +504,25 @@ __global__ void __launch_bounds__(64) k_clip_compact(const int32_t *__restrict__
                                                      int clip_is_dst, const double *__restrict__ a_dst,
                                                      const double *__restrict__ a_src, double thr,
                                                      double *__restrict__ val) {
-    __shared__ int32_t queue[128];
+    static_assert(NC == 4, "stage queues are for 4-vertex clip polygons");
+    // queues of candidates that cross 1-2, 3 and 4 clip edges (a pipeline stage
+    // per crossed edge: about 2.3 stages instead of 4 on the benchmark grids)
+    __shared__ int32_t queue[3][128];
     const int lane = threadIdx.x;
     const int64_t base = (int64_t)blockIdx.x * (64 * kClipChunks);
-    int qn = 0;
+    int qn0 = 0, qn1 = 0, qn2 = 0;
     for (int ch = 0; ch <= kClipChunks; ++ch) {
         if (ch < kClipChunks) {
             const int64_t c = base + ch * 64 + lane;
             bool keep = false;
+            unsigned crossed = 0;
             if (c < ncand) {
                 const int64_t d = cand_dst[c], s = cand_src[c];
                 const double *dp = dst_cells + 3 * (int64_t)dst_nv * d;
                 const double *sp = src_cells + 3 * (int64_t)src_nv * s;
                 const double *clip = clip_is_dst ? dp : sp, *subj = clip_is_dst ? sp : dp;
                 const int nc = clip_is_dst ? dst_nv : src_nv, ns = clip_is_dst ? src_nv : dst_nv;
-                const int where = classify_subject<NC>(clip, nc, subj, ns);
+                const int where = classify_subject<NC>(clip, nc, subj, ns, &crossed);
                 keep = where == kSubjectCrosses;
                 if (where == kSubjectOutside) val[c] = 0.0;
                 if (where == kSubjectInside) {  // the subject's own (unit-sphere) area
@@ -490,24 +530,32 @@ __global__ void __launch_bounds__(64) k_clip_compact(const int32_t *__restrict__
                     val[c] = kept_area(a, fmin(a_src[s], a_dst[d]), thr);
                 }
             }
-            const uint64_t m = __ballot(keep);
-            if (keep) queue[qn + (int)__popcll(m & ((1ull << lane) - 1ull))] = ch * 64 + lane;
-            qn += (int)__popcll(m);
+            const int stages = __builtin_popcount(crossed);
+            const int qi = stages <= 2 ? 0 : stages - 2;
+            const int32_t entry = (ch * 64 + lane) | (int32_t)(crossed << 16);
+            const uint64_t below = (1ull << lane) - 1ull;
+            const uint64_t m0 = __ballot(keep && qi == 0), m1 = __ballot(keep && qi == 1),
+                           m2 = __ballot(keep && qi == 2);
+            if (keep && qi == 0) queue[0][qn0 + (int)__popcll(m0 & below)] = entry;
+            if (keep && qi == 1) queue[1][qn1 + (int)__popcll(m1 & below)] = entry;
+            if (keep && qi == 2) queue[2][qn2 + (int)__popcll(m2 & below)] = entry;
+            qn0 += (int)__popcll(m0);
+            qn1 += (int)__popcll(m1);
+            qn2 += (int)__popcll(m2);
             __syncthreads();
         }
-        // a full batch, or what is left after the last chunk (one call site:
-        // the pipeline's code exists once)
-        if (qn >= 64 || (ch == kClipChunks && qn > 0)) {
-            if (lane < qn)
-                clip_candidate<NC>(base + queue[lane], cand_dst, cand_src, dst_cells, dst_nv, src_cells, src_nv,
-                                   clip_is_dst, a_dst, a_src, thr, val);
-            const int left = qn > 64 ? qn - 64 : 0;
-            const int moved = lane < left ? queue[64 + lane] : 0;
-            __syncthreads();
-            if (lane < left) queue[lane] = moved;
-            qn = left;
-            __syncthreads();
-        }
+        // full batches; after the last chunk the short queues' remainders move
+        // up (a longer pipeline clips them too) and the 4-stage queue drains.
+        // One call site per pipeline length.
+        const bool last = ch == kClipChunks;
+        if (last) promote(queue[0], qn0, queue[1], qn1);
+        clip_queued<2, NC>(queue[0], qn0, false, base, cand_dst, cand_src, dst_cells, dst_nv, src_cells, src_nv,
+                           clip_is_dst, a_dst, a_src, thr, val);
+        clip_queued<3, NC>(queue[1], qn1, false, base, cand_dst, cand_src, dst_cells, dst_nv, src_cells, src_nv,
+                           clip_is_dst, a_dst, a_src, thr, val);
+        if (last) promote(queue[1], qn1, queue[2], qn2);
+        clip_queued<4, NC>(queue[2], qn2, last, base, cand_dst, cand_src, dst_cells, dst_nv, src_cells, src_nv,
+                           clip_is_dst, a_dst, a_src, thr, val);
     }
 }
 

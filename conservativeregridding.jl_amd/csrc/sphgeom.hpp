@@ -189,6 +189,42 @@ CR_HD double clip_area_stream(const double *clip, int nc, const double *subj, in
     return P.acc.area();
 }
 
+// The same with a pipeline of NS stages, one per clip edge whose bit is set in
+// `mask` (exactly NS bits, in edge order).  An edge that has every subject
+// vertex on its inside passes the clipped polygon through unchanged (the
+// polygon stays inside the subject, which is inside that edge's hemisphere),
+// so dropping its stage changes nothing but rounding.
+template <int K, int NS, int NC>
+CR_HD void pipe_set_masked(ClipPipe<NS> &P, const double *clip, int nc, unsigned mask) {
+    if constexpr (K < NS) {
+        const int e = __builtin_ctz(mask), e1 = (e + 1) % NC;
+        P.set_edge(K, load3(clip + 3 * (e < nc ? e : nc - 1)), load3(clip + 3 * (e1 < nc ? e1 : nc - 1)));
+        pipe_set_masked<K + 1, NS, NC>(P, clip, nc, mask & (mask - 1));
+    }
+}
+
+template <int NS, int NC>
+CR_HD double clip_area_stream_masked(const double *clip, int nc, unsigned mask, const double *subj, int ns) {
+    ClipPipe<NS> P;
+    P.init();
+    pipe_set_masked<0, NS, NC>(P, clip, nc, mask);
+    d3 v0 = load3(subj), v1 = load3(subj + 3 * (ns > 1 ? 1 : 0));
+    for (int i = 0; i < ns; ++i) {
+        const d3 v2 = load3(subj + 3 * (i + 2 < ns ? i + 2 : ns - 1));
+        pipe_push<0, NS>(P, v0);
+        v0 = v1;
+        v1 = v2;
+    }
+    pipe_close<0, NS>(P);
+    return P.acc.area();
+}
+
+// `mask` with its lowest clear bits set until it has n bits (n <= NC).
+CR_HD unsigned pad_mask(unsigned mask, int n) {
+    while (__builtin_popcount(mask) < n) mask |= ~mask & (mask + 1u);
+    return mask;
+}
+
 // Where the subject lies against the clip polygon, by the pipeline's own side
 // test n.(v - a) (a degenerate edge has n = 0: everything is on its inside):
 // kSubjectOutside when every subject vertex is strictly outside one clip edge
@@ -198,7 +234,7 @@ CR_HD double clip_area_stream(const double *clip, int nc, const double *subj, in
 enum { kSubjectOutside = 0, kSubjectInside = 1, kSubjectCrosses = 2 };
 
 template <int NC>
-CR_HD int classify_subject(const double *clip, int nc, const double *subj, int ns) {
+CR_HD int classify_subject(const double *clip, int nc, const double *subj, int ns, unsigned *crossed = nullptr) {
     d3 a[NC], n[NC];
 #pragma unroll
     for (int k = 0; k < NC; ++k) a[k] = load3(clip + 3 * (k < nc ? k : nc - 1));
@@ -211,15 +247,18 @@ CR_HD int classify_subject(const double *clip, int nc, const double *subj, int n
         n[k] = nk;
     }
     bool in = true;
+    unsigned cut = 0;  // edges with a subject vertex outside
     for (int i = 0; i < ns; ++i) {
         const d3 v = load3(subj + 3 * i);
 #pragma unroll
         for (int k = 0; k < NC; ++k) {
             const double s = dot(n[k], v - a[k]);
             out[k] = out[k] && s < 0.0;
-            in = in && s >= 0.0;
+            cut |= (s < 0.0 ? 1u : 0u) << k;
         }
     }
+    in = cut == 0;
+    if (crossed) *crossed = cut;
     bool any = false;
 #pragma unroll
     for (int k = 0; k < NC; ++k) any = any || out[k];
