@@ -39,11 +39,15 @@ __global__ void k_pair_coef(const int64_t *__restrict__ rowptr, const int32_t *_
 }
 
 // Scatter target of the clip backward: caller-vertex gradients of both grids.
+// The destination cell's slots may go to a block-local accumulator instead
+// (lacc_subj or lacc_clip, 3 doubles per slot), flushed once per block.
 struct VertexSink {
     const double *subj, *clip;
     int ns, nc;
     const int32_t *subj_vi, *clip_vi;  // caller vertex of each slot of the two cells
     double *g_subj, *g_clip;           // caller-vertex gradients (may be null)
+    double *lacc_subj, *lacc_clip;     // LDS accumulators of one side's slots, or null
+    double *own_clip;                  // the pair's own 4 x 3 clip-slot sums in LDS, or null
 
     __device__ d3 sv(int e) const { return load3(subj + 3 * (e % ns)); }
     __device__ d3 cv(int k) const {
@@ -56,14 +60,29 @@ struct VertexSink {
         atomicAdd(g + 3 * (int64_t)v + 1, d.y);
         atomicAdd(g + 3 * (int64_t)v + 2, d.z);
     }
+    __device__ static void add_local(double *l, int slot, const d3 &d) {
+        atomicAdd(l + 3 * slot, d.x);
+        atomicAdd(l + 3 * slot + 1, d.y);
+        atomicAdd(l + 3 * slot + 2, d.z);
+    }
     __device__ void add_subj(int e, const d3 &g) const {
         e %= ns;
-        add(g_subj, subj_vi[e], tangential(g, sv(e)));
+        if (lacc_subj) add_local(lacc_subj, e, tangential(g, sv(e)));
+        else add(g_subj, subj_vi[e], tangential(g, sv(e)));
     }
     __device__ void add_clip(int k, const d3 &g) const {
         int kk = k % 4;
         kk = kk < nc ? kk : nc - 1;
-        add(g_clip, clip_vi[kk], tangential(g, cv(kk)));
+        const d3 t = tangential(g, cv(kk));
+        if (lacc_clip) {
+            add_local(lacc_clip, kk, t);
+        } else if (own_clip) {  // this pair's own sums (no other thread touches them)
+            own_clip[3 * kk] += t.x;
+            own_clip[3 * kk + 1] += t.y;
+            own_clip[3 * kk + 2] += t.z;
+        } else {
+            add(g_clip, clip_vi[kk], t);
+        }
     }
     __device__ void operator()(int32_t l, const d3 &x, const d3 &g) const {
         if (l < 0) return;
@@ -85,27 +104,58 @@ struct VertexSink {
     }
 };
 
-__global__ void __launch_bounds__(256) k_clip_grad(
+// One wave per 64 consecutive non-zeros.  CSR rows are destination cells, so a
+// wave's pairs share few destination cells (about 12 on the benchmark): their
+// slots' gradients are summed in LDS (rows r0 .. r0 + kGradRows - 1 of the
+// wave) and reach memory once per wave.  A source cell that clips has its 4
+// slots summed per pair and written once per pair; other contributions use
+// global FP64 atomics directly.
+constexpr int kGradRows = 16;
+
+__global__ void __launch_bounds__(64) k_clip_grad(
     const int32_t *__restrict__ rowid, const int32_t *__restrict__ col, const double *__restrict__ coef,
     int64_t nnz, const double *__restrict__ dst_cells, int dst_nv, const int32_t *__restrict__ dst_vi,
     const double *__restrict__ src_cells, int src_nv, const int32_t *__restrict__ src_vi,
     int clip_is_dst, double *__restrict__ g_dst, double *__restrict__ g_src) {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nnz) return;
-    double c = coef[k];
-    if (c == 0.0) return;
-    int64_t d = rowid[k], s = col[k];
-    const double *dp = dst_cells + 3 * (int64_t)dst_nv * d;
-    const double *sp = src_cells + 3 * (int64_t)src_nv * s;
-    VertexSink sink;
-    if (clip_is_dst) {
-        sink = VertexSink{sp, dp, src_nv, dst_nv, src_vi + (int64_t)src_nv * s, dst_vi + (int64_t)dst_nv * d,
-                          g_src, g_dst};
-    } else {
-        sink = VertexSink{dp, sp, dst_nv, src_nv, dst_vi + (int64_t)dst_nv * d, src_vi + (int64_t)src_nv * s,
-                          g_dst, g_src};
+    extern __shared__ double lacc[];  // kGradRows * dst_nv * 3
+    __shared__ double own[64 * 12];
+    const int64_t k0 = (int64_t)blockIdx.x * 64, k = k0 + threadIdx.x;
+    const int64_t r0 = rowid[k0];
+    const int nacc = kGradRows * dst_nv * 3;
+    for (int i = threadIdx.x; i < nacc; i += 64) lacc[i] = 0.0;
+    double *my_own = own + 12 * threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) my_own[i] = 0.0;
+    __syncthreads();
+    if (k < nnz && coef[k] != 0.0) {
+        const double c = coef[k];
+        const int64_t d = rowid[k], s = col[k];
+        const double *dp = dst_cells + 3 * (int64_t)dst_nv * d;
+        const double *sp = src_cells + 3 * (int64_t)src_nv * s;
+        double *mine = (g_dst && d - r0 < kGradRows) ? lacc + (d - r0) * dst_nv * 3 : nullptr;
+        VertexSink sink;
+        if (clip_is_dst) {
+            sink = VertexSink{sp, dp, src_nv, dst_nv, src_vi + (int64_t)src_nv * s, dst_vi + (int64_t)dst_nv * d,
+                              g_src, g_dst, nullptr, mine, nullptr};
+        } else {
+            sink = VertexSink{dp, sp, dst_nv, src_nv, dst_vi + (int64_t)dst_nv * d, src_vi + (int64_t)src_nv * s,
+                              g_dst, g_src, mine, nullptr, g_src ? my_own : nullptr};
+        }
+        clip_area_backward<4, VertexSink>(sink.clip, sink.nc, sink.subj, sink.ns, c, sink);
+        if (sink.own_clip)
+            for (int kk = 0; kk < src_nv && kk < 4; ++kk)
+                VertexSink::add(g_src, sink.clip_vi[kk], mk3(my_own[3 * kk], my_own[3 * kk + 1], my_own[3 * kk + 2]));
     }
-    clip_area_backward<4, VertexSink>(sink.clip, sink.nc, sink.subj, sink.ns, c, sink);
+    __syncthreads();
+    if (!g_dst) return;
+    const int64_t klast = min(nnz, k0 + 64) - 1;
+    const int64_t nrows = min((int64_t)kGradRows, (int64_t)rowid[klast] - r0 + 1);
+    for (int i = threadIdx.x; i < nrows * dst_nv * 3; i += 64) {
+        const double v = lacc[i];
+        if (v == 0.0) continue;
+        const int r = i / (dst_nv * 3), slot = (i / 3) % dst_nv, comp = i % 3;
+        atomicAdd(g_dst + 3 * (int64_t)dst_vi[(r0 + r) * dst_nv + slot] + comp, v);
+    }
 }
 
 // d(area of each destination cell)/dV times dL/da_u.
@@ -186,7 +236,8 @@ cr_status cr_regrid_backward(const cr_regridder *R, const cr_grid *dst, const cr
                            (int)(transpose != 0), (int)(normalize != 0), kappa, a_to, nfields, x, xc, xf, gy,
                            gc, gf, coef.p, rowid.p);
         if (F.nnz)
-            hipLaunchKernelGGL(k_clip_grad, dim3(grid_for(F.nnz, B)), dim3(B), 0, s, rowid.p, F.col.p, coef.p,
+            hipLaunchKernelGGL(k_clip_grad, dim3(grid_for(F.nnz, 64)), dim3(64),
+                               (size_t)kGradRows * dst->nv * 3 * sizeof(double), s, rowid.p, F.col.p, coef.p,
                                F.nnz, dst->cells.p, dst->nv, dst->vidx.p, src->cells.p, src->nv, src->vidx.p,
                                R->stats.clip_with_dst, g_dst_verts, g_src_verts);
         double *g_to = transpose ? g_src_verts : g_dst_verts;
