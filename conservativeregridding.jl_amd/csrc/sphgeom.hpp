@@ -462,21 +462,26 @@ CR_HD bool crossing_backward(const d3 &x, const d3 &g, const d3 &p1, const d3 &q
     return true;
 }
 
-// coef * d(signed ring area)/d(vertices), one call of emit(slot, grad) per vertex use.
+// coef * d(signed ring area)/d(vertices) through emit(slot, grad): one call per
+// vertex (a vertex's parts from the two fan triangles that share it are summed
+// before the call).
 template <class Emit>
 CR_HD void ring_area_backward(const double *xyz, int n, double coef, Emit &&emit) {
     if (n < 3) return;
     d3 a = load3(xyz);
-    d3 ga = mk3(0, 0, 0);
+    d3 ga = mk3(0, 0, 0), pend = mk3(0, 0, 0);
+    d3 b = load3(xyz + 3);
     for (int i = 1; i + 1 < n; ++i) {
-        d3 b = load3(xyz + 3 * i), c = load3(xyz + 3 * (i + 1));
+        const d3 c = load3(xyz + 3 * (i + 1));
         double T = dot(a, cross(b, c));
         double D = 1.0 + dot(a, b) + dot(b, c) + dot(c, a);
         double f = coef * 2.0 / (T * T + D * D);
         ga = ga + f * (D * cross(b, c) - T * (b + c));
-        emit(i, tangential(f * (D * cross(c, a) - T * (a + c)), b));
-        emit(i + 1, tangential(f * (D * cross(a, b) - T * (a + b)), c));
+        emit(i, tangential(pend + f * (D * cross(c, a) - T * (a + c)), b));
+        pend = f * (D * cross(a, b) - T * (a + b));
+        b = c;
     }
+    emit(n - 1, tangential(pend, b));
     emit(0, tangential(ga, a));
 }
 
