@@ -223,6 +223,104 @@ __device__ int coop_collect(const float4 &wc, const float4 *__restrict__ nodes, 
     return nf;
 }
 
+// The whole wave walks the tree once with the four 16-lane cluster caps
+// gc[0..3] (wave-uniform): the frontier is the union of what touches any of
+// them, and the leaf level lists, for each cluster g, the leaves that touch
+// gc[g] in lcap/lid[g * kSubLeaves ...] (n[g] of them, in tree order).  Each
+// lane then tests only its own cluster's list, a quarter of the lanes' region
+// and far fewer leaves than one 64-cell list.  Returns false when the union
+// frontier or a list overflows or a cap cannot be bounded; an empty cluster
+// (w < 0) gets an empty list.
+constexpr int kSubLeaves = kSearchLeaves / 4;
+
+__device__ bool coop_collect4(const float4 (&gc)[4], const float4 *__restrict__ nodes, const Levels &lv,
+                              const int32_t *__restrict__ perm, float4 *lcap, int32_t *lid,
+                              int32_t (*front)[kSearchFront], int (&n)[4]) {
+    const int lane = threadIdx.x;
+    bool live[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        if (gc[g].w >= 1.4f) return false;
+        live[g] = gc[g].w >= 0.0f;
+        n[g] = 0;
+    }
+    if (lv.top == 0) {  // a single leaf
+        __syncthreads();
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+            if (live[g] && caps_overlap(gc[g], nodes[0])) {
+                if (lane == 0) {
+                    lcap[g * kSubLeaves] = nodes[0];
+                    lid[g * kSubLeaves] = perm[0];
+                }
+                n[g] = 1;
+            }
+        __syncthreads();
+        return true;
+    }
+    __syncthreads();  // the previous lists are no longer read
+    if (lane == 0) front[0][0] = 0;
+    __syncthreads();
+    int cur = 0, nf = 1;
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int lev = lv.top; lev >= 1; --lev) {
+        const int64_t csize = lv.size[lev - 1], coff = lv.off[lev - 1];
+        const int tot = nf * kTreeFanout;
+        int nn = 0;
+        for (int base = 0; base < tot; base += 4 * 64) {
+            int64_t child[4];
+            float4 cc[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int t = base + 64 * u + lane;
+                child[u] = -1;
+                cc[u] = make_float4(0.f, 0.f, 0.f, -8.0f);
+                if (t < tot) {
+                    const int64_t ch = (int64_t)front[cur][t >> 3] * kTreeFanout + (t & 7);
+                    if (ch < csize) {
+                        child[u] = ch;
+                        cc[u] = nodes[coff + ch];
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                bool h[4];
+#pragma unroll
+                for (int g = 0; g < 4; ++g) h[g] = child[u] >= 0 && live[g] && caps_overlap(gc[g], cc[u]);
+                if (lev > 1) {
+                    const bool hit = h[0] || h[1] || h[2] || h[3];
+                    const uint64_t m = __ballot(hit);
+                    const int pos = nn + (int)__popcll(m & below);
+                    if (hit && pos < kSearchFront) front[cur ^ 1][pos] = (int32_t)child[u];
+                    nn += (int)__popcll(m);
+                } else {
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        const uint64_t m = __ballot(h[g]);
+                        const int pos = n[g] + (int)__popcll(m & below);
+                        if (h[g] && pos < kSubLeaves) {
+                            lcap[g * kSubLeaves + pos] = cc[u];
+                            lid[g * kSubLeaves + pos] = perm[child[u]];
+                        }
+                        n[g] += (int)__popcll(m);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (lev > 1) {
+            if (nn > kSearchFront) return false;
+            cur ^= 1;
+            nf = nn;
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+        if (n[g] > kSubLeaves) return false;
+    return true;
+}
+
 // One lane's tests against the collected list (four independent tests per
 // step: a wave per SIMD has no other wave to hide a dependent chain).
 template <bool FILL>
@@ -378,7 +476,8 @@ __global__ void __launch_bounds__(64) k_search(const float4 *__restrict__ qcap,
                                                const int32_t *__restrict__ perm,
                                                int64_t *__restrict__ count_or_offset,
                                                int32_t *__restrict__ cand_src,
-                                               int32_t *__restrict__ cand_dst) {
+                                               int32_t *__restrict__ cand_dst,
+                                               int32_t *__restrict__ defer, unsigned *__restrict__ ndefer) {
     __shared__ float4 lcap[kSearchLeaves];
     __shared__ int32_t lid[kSearchLeaves];
     __shared__ int32_t front[2][kSearchFront];
@@ -388,16 +487,25 @@ __global__ void __launch_bounds__(64) k_search(const float4 *__restrict__ qcap,
     const int64_t d = active ? qorder[i] : 0;
     const float4 c = active ? qcap[d] : make_float4(0.f, 0.f, 0.f, 0.f);
 
-    // the 64-cell cluster
-    const float4 wc = group_cap<64>(c, active);
-    int n = coop_collect(wc, nodes, lv, perm, lcap, lid, front);
-    if (n >= 0) {
-        if (active) lane_emit<FILL>(c, d, n, lcap, lid, count_or_offset, cand_src, cand_dst);
-        return;
-    }
-    // four 16-cell clusters, one after the other; the lanes of a cluster that
-    // still overflows walk the tree alone, all together at the end
+    // four 16-cell clusters in one walk
     const float4 gcap = group_cap<16>(c, active);
+    {
+        float4 gcs[4];
+        int n4[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) gcs[g] = shfl4(gcap, 16 * g);
+        if (coop_collect4(gcs, nodes, lv, perm, lcap, lid, front, n4)) {
+            const int g = lane >> 4;
+            const int n = g == 0 ? n4[0] : (g == 1 ? n4[1] : (g == 2 ? n4[2] : n4[3]));
+            if (active)
+                lane_emit<FILL>(c, d, n, lcap + g * kSubLeaves, lid + g * kSubLeaves, count_or_offset, cand_src,
+                                cand_dst);
+            return;
+        }
+    }
+    // the four clusters one after the other with full-size lists; the lanes of
+    // a cluster that still overflows walk the tree alone, all together at the end
+    int n;
     bool walk = false;
     for (int g = 0; g < 4; ++g) {
         const float4 gc = shfl4(gcap, 16 * g);
@@ -409,19 +517,45 @@ __global__ void __launch_bounds__(64) k_search(const float4 *__restrict__ qcap,
             walk = true;
         }
     }
-    // cells left over: one at a time, each walked by the whole wave
-    uint64_t left = __ballot(walk && active);
-    bool alone = false;
-    while (left) {
-        const int L = (int)__ffsll((long long)left) - 1;
-        left &= left - 1;
-        const float4 qc = shfl4(c, L);
-        const int64_t dq = (int64_t)__shfl((int)d, L, 64);
-        if (!coop_emit_one<FILL>(qc, dq, nodes, lv, perm, (int32_t *)lcap, count_or_offset, cand_src, cand_dst) &&
-            lane == L)
-            alone = true;
+    // cells left over (about 0.15% of them, with hundreds to thousands of
+    // candidates each): listed by the count pass for k_search_deferred, which
+    // gives each its own wave (walked here one at a time they would hold this
+    // wave for a long tail)
+    if (!FILL) {
+        const bool mine = walk && active;
+        const uint64_t m = __ballot(mine);
+        if (m) {
+            const int leader = (int)__ffsll((long long)m) - 1;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(ndefer, (unsigned)__popcll(m));
+            base = (unsigned)__shfl((int)base, leader, 64);
+            if (mine) defer[base + (unsigned)__popcll(m & ((1ull << lane) - 1ull))] = (int32_t)d;
+        }
     }
-    if (alone) lane_walk<FILL>(c, d, nodes, lv, perm, count_or_offset, cand_src, cand_dst);
+}
+
+// The listed cells, one wave each (grid-stride over the list): the whole wave
+// walks the tree with the cell's cap and counts or writes the hits straight
+// from its ballots; a cell whose frontier overflows even then is walked by
+// one lane alone (tree_query).
+template <bool FILL>
+__global__ void __launch_bounds__(64) k_search_deferred(const float4 *__restrict__ qcap,
+                                                        const int32_t *__restrict__ defer,
+                                                        const unsigned *__restrict__ ndefer,
+                                                        const float4 *__restrict__ nodes, Levels lv,
+                                                        const int32_t *__restrict__ perm,
+                                                        int64_t *__restrict__ count_or_offset,
+                                                        int32_t *__restrict__ cand_src,
+                                                        int32_t *__restrict__ cand_dst) {
+    __shared__ int32_t scratch[kSearchBigFront];
+    const unsigned n = *ndefer;
+    for (unsigned i = blockIdx.x; i < n; i += gridDim.x) {
+        const int64_t d = defer[i];
+        const float4 qc = qcap[d];
+        if (!coop_emit_one<FILL>(qc, d, nodes, lv, perm, scratch, count_or_offset, cand_src, cand_dst) &&
+            threadIdx.x == 0)
+            lane_walk<FILL>(qc, d, nodes, lv, perm, count_or_offset, cand_src, cand_dst);
+    }
 }
 
 // Candidates whose subject lies outside one clip edge (about a quarter on the
@@ -864,6 +998,10 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     DevBuf<int64_t> cand_off(nd + 1, s);
     DevBuf<int32_t> qorder(nd, s);
     const unsigned qblocks = grid_for(nd, 64);
+    DevBuf<int32_t> defer(nd, s);  // cells whose clusters overflow, listed by the count pass
+    DevBuf<unsigned> ndefer(1, s);
+    CR_HIP(hipMemsetAsync(ndefer.p, 0, sizeof(unsigned), s));
+    constexpr unsigned kDeferBlocks = 2048;
     {
         DevBuf<uint32_t> key(nd, s), key_sorted(nd, s);
         DevBuf<int32_t> idx(nd, s);
@@ -872,7 +1010,9 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
         DevBuf<int64_t> cand_cnt(nd + 1, s);
         CR_HIP(hipMemsetAsync(cand_cnt.p + nd, 0, sizeof(int64_t), s));
         hipLaunchKernelGGL(k_search<false>, dim3(qblocks), dim3(64), 0, s, cap_dst.p, qorder.p, nd, nodes.p,
-                           lv, perm.p, cand_cnt.p, (int32_t *)nullptr, (int32_t *)nullptr);
+                           lv, perm.p, cand_cnt.p, (int32_t *)nullptr, (int32_t *)nullptr, defer.p, ndefer.p);
+        hipLaunchKernelGGL(k_search_deferred<false>, dim3(kDeferBlocks), dim3(64), 0, s, cap_dst.p, defer.p,
+                           ndefer.p, nodes.p, lv, perm.p, cand_cnt.p, (int32_t *)nullptr, (int32_t *)nullptr);
         CR_HIP(hipGetLastError());
         exclusive_sum(T, cand_cnt.p, cand_off.p, nd + 1, s);
     }
@@ -885,9 +1025,12 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
         CR_HIP(hipMemsetAsync(cand_src.p, 0, cand_src.bytes(), s));
         CR_HIP(hipMemsetAsync(cand_dst.p, 0, cand_dst.bytes(), s));
     }
-    if (ncand)
+    if (ncand) {
         hipLaunchKernelGGL(k_search<true>, dim3(qblocks), dim3(64), 0, s, cap_dst.p, qorder.p, nd, nodes.p,
-                           lv, perm.p, cand_off.p, cand_src.p, cand_dst.p);
+                           lv, perm.p, cand_off.p, cand_src.p, cand_dst.p, defer.p, ndefer.p);
+        hipLaunchKernelGGL(k_search_deferred<true>, dim3(kDeferBlocks), dim3(64), 0, s, cap_dst.p, defer.p,
+                           ndefer.p, nodes.p, lv, perm.p, cand_off.p, cand_src.p, cand_dst.p);
+    }
     CR_HIP(hipGetLastError());
     tm.mark();  // 3
 
