@@ -78,11 +78,13 @@ void launch_cell_bounds(const cr_grid *g, double *area, float4 *cap, hipStream_t
     }
 }
 
-__global__ void k_sfc_keys(const float4 *__restrict__ cap, int64_t n, uint32_t *__restrict__ key,
-                         int32_t *__restrict__ idx) {
+// Curve keys of n caps (with `tag` or-ed in: the top bit tells grids apart in
+// a combined sort) and the identity permutation.
+__global__ void k_sfc_keys(const float4 *__restrict__ cap, int64_t n, uint32_t tag, uint32_t *__restrict__ key,
+                           int32_t *__restrict__ idx) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
-    key[c] = sfc_key(cap[c]);
+    key[c] = sfc_key(cap[c]) | tag;
     idx[c] = (int32_t)c;
 }
 
@@ -975,17 +977,23 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     CR_HIP(hipGetLastError());
     tm.mark();  // 1
 
-    // 2. tree over source caps
+    // 2. tree over source caps; one sort of both grids' curve keys (source
+    // keys first: the destination keys carry the top bit) gives the tree order
+    // of the source cells and the query order of the destination cells
+    CR_REQUIRE(ns + nd < (int64_t(1) << 31), CR_UNSUPPORTED, "more than 2^31-1 cells in the two grids");
     Levels lv = make_levels(ns);
     int64_t nnodes = lv.off[lv.top] + 1;
     DevBuf<float4> nodes(nnodes, s);
-    DevBuf<int32_t> perm(ns, s);
+    DevBuf<int32_t> order(ns + nd, s);
+    const int32_t *perm = order.p, *qorder = order.p + ns;
     {
-        DevBuf<uint32_t> key(ns, s), key_sorted(ns, s);
-        DevBuf<int32_t> idx(ns, s);
-        hipLaunchKernelGGL(k_sfc_keys, dim3(grid_for(ns, B)), dim3(B), 0, s, cap_src.p, ns, key.p, idx.p);
-        sort_pairs(T, key.p, key_sorted.p, idx.p, perm.p, ns, 2 * kSfcOrder + 3, s);
-        hipLaunchKernelGGL(k_gather_caps, dim3(grid_for(ns, B)), dim3(B), 0, s, cap_src.p, perm.p, ns,
+        DevBuf<uint32_t> key(ns + nd, s), key_sorted(ns + nd, s);
+        DevBuf<int32_t> idx(ns + nd, s);
+        hipLaunchKernelGGL(k_sfc_keys, dim3(grid_for(ns, B)), dim3(B), 0, s, cap_src.p, ns, 0u, key.p, idx.p);
+        hipLaunchKernelGGL(k_sfc_keys, dim3(grid_for(nd, B)), dim3(B), 0, s, cap_dst.p, nd, 1u << 31, key.p + ns,
+                           idx.p + ns);
+        sort_pairs(T, key.p, key_sorted.p, idx.p, order.p, ns + nd, 32, s);
+        hipLaunchKernelGGL(k_gather_caps, dim3(grid_for(ns, B)), dim3(B), 0, s, cap_src.p, perm, ns,
                            nodes.p);
         for (int L = 1; L <= lv.top; ++L)
             hipLaunchKernelGGL(k_build_level, dim3(grid_for(lv.size[L], B)), dim3(B), 0, s,
@@ -996,23 +1004,18 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
 
     // 3. candidate search: destination cells in curve order, count, scan, fill
     DevBuf<int64_t> cand_off(nd + 1, s);
-    DevBuf<int32_t> qorder(nd, s);
     const unsigned qblocks = grid_for(nd, 64);
     DevBuf<int32_t> defer(nd, s);  // cells whose clusters overflow, listed by the count pass
     DevBuf<unsigned> ndefer(1, s);
     CR_HIP(hipMemsetAsync(ndefer.p, 0, sizeof(unsigned), s));
     constexpr unsigned kDeferBlocks = 2048;
     {
-        DevBuf<uint32_t> key(nd, s), key_sorted(nd, s);
-        DevBuf<int32_t> idx(nd, s);
-        hipLaunchKernelGGL(k_sfc_keys, dim3(grid_for(nd, B)), dim3(B), 0, s, cap_dst.p, nd, key.p, idx.p);
-        sort_pairs(T, key.p, key_sorted.p, idx.p, qorder.p, nd, 2 * kSfcOrder + 3, s);
         DevBuf<int64_t> cand_cnt(nd + 1, s);
         CR_HIP(hipMemsetAsync(cand_cnt.p + nd, 0, sizeof(int64_t), s));
-        hipLaunchKernelGGL(k_search<false>, dim3(qblocks), dim3(64), 0, s, cap_dst.p, qorder.p, nd, nodes.p,
-                           lv, perm.p, cand_cnt.p, (int32_t *)nullptr, (int32_t *)nullptr, defer.p, ndefer.p);
+        hipLaunchKernelGGL(k_search<false>, dim3(qblocks), dim3(64), 0, s, cap_dst.p, qorder, nd, nodes.p,
+                           lv, perm, cand_cnt.p, (int32_t *)nullptr, (int32_t *)nullptr, defer.p, ndefer.p);
         hipLaunchKernelGGL(k_search_deferred<false>, dim3(kDeferBlocks), dim3(64), 0, s, cap_dst.p, defer.p,
-                           ndefer.p, nodes.p, lv, perm.p, cand_cnt.p, (int32_t *)nullptr, (int32_t *)nullptr);
+                           ndefer.p, nodes.p, lv, perm, cand_cnt.p, (int32_t *)nullptr, (int32_t *)nullptr);
         CR_HIP(hipGetLastError());
         exclusive_sum(T, cand_cnt.p, cand_off.p, nd + 1, s);
     }
@@ -1026,10 +1029,10 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
         CR_HIP(hipMemsetAsync(cand_dst.p, 0, cand_dst.bytes(), s));
     }
     if (ncand) {
-        hipLaunchKernelGGL(k_search<true>, dim3(qblocks), dim3(64), 0, s, cap_dst.p, qorder.p, nd, nodes.p,
-                           lv, perm.p, cand_off.p, cand_src.p, cand_dst.p, defer.p, ndefer.p);
+        hipLaunchKernelGGL(k_search<true>, dim3(qblocks), dim3(64), 0, s, cap_dst.p, qorder, nd, nodes.p,
+                           lv, perm, cand_off.p, cand_src.p, cand_dst.p, defer.p, ndefer.p);
         hipLaunchKernelGGL(k_search_deferred<true>, dim3(kDeferBlocks), dim3(64), 0, s, cap_dst.p, defer.p,
-                           ndefer.p, nodes.p, lv, perm.p, cand_off.p, cand_src.p, cand_dst.p);
+                           ndefer.p, nodes.p, lv, perm, cand_off.p, cand_src.p, cand_dst.p);
     }
     CR_HIP(hipGetLastError());
     tm.mark();  // 3
