@@ -138,3 +138,26 @@ def test_transposed_csr_is_consistent(cr):
     assert At.shape == (A.shape[1], A.shape[0])
     assert (At - A.T.tocsr()).nnz == 0 or np.max(np.abs((At - A.T).data)) == 0.0
     assert cr.transpose(R).src_areas is R.dst_areas
+
+
+@pytest.mark.parametrize("coarse_dst", [True, False])
+def test_coarse_cells_with_thousands_of_candidates(cr, orc, coarse_dst):
+    """HEALPix nside 2 (48 cells of ~30 deg) against a 1 deg lon-lat grid: each
+    coarse cell touches over a thousand fine cells, more than a search list
+    holds, so the search's overflow routes (per-cluster walks, cells walked by
+    a wave of their own) carry the candidates.  Sums of A are the cell areas."""
+    _print_case("coarse_fine", coarse_dst=coarse_dst)
+    ll = cr.lonlat_points(360, 180)
+    hp = cr.Grid.healpix(2, "nested")
+    if coarse_dst:
+        R = cr.Regridder(hp, cr.Grid.from_points(ll), radius=1.0)
+        ref = orc.build(hp.cells(), cr.points_to_cells(ll))
+    else:
+        R = cr.Regridder(cr.Grid.from_points(ll), hp, radius=1.0)
+        ref = orc.build(cr.points_to_cells(ll), hp.cells())
+    pattern_and_values(R, ref)
+    A = R.intersections
+    np.testing.assert_allclose(np.asarray(A.sum(1)).ravel(), R.dst_areas, rtol=1e-12)
+    np.testing.assert_allclose(np.asarray(A.sum(0)).ravel(), R.src_areas, rtol=1e-12)
+    if coarse_dst:
+        assert np.diff(A.indptr).min() > 256 and np.diff(A.indptr).max() > 1024
