@@ -109,6 +109,8 @@ SIGNATURES = {
     "cr_esmf_write": (c_i32, [ctypes.c_char_p, c_i64, _dp, _i32p, _i32p, c_i64, _dp, _dp, c_i64, _dp, _dp,
                               ctypes.POINTER(EsmfAttrs)]),
     "cr_regrid_set_variant": (c_i32, [c_vp, c_i32, c_i32, c_i32]),
+    "cr_regrid_tile_plan": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp]),
+    "cr_regrid_copy_tile_plan": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
 }
 
 

@@ -48,8 +48,8 @@ struct Csr {
     mutable std::mutex plan_mu;
 };
 
-// The tile plan of A in the given TileStyle and TileGeomId, built on the first
-// call (synchronises stream s once).
+// The tile plan of A in the given TileStyle and TileGeomId, made on the device
+// on the first call (synchronises stream s).
 const TilePlan &tile_plan(const Csr &A, int style, int geom, hipStream_t s);
 // Forget the plan (after A's values change).
 void drop_tile_plan(const Csr &A);

@@ -12,6 +12,7 @@
 //     64 pieces of a round padded to the longest (tileplan.hpp): long and
 //     short rows cost the same per thread, and every lane of a wave ends its
 //     segments on the same slot (a wave-uniform mask, no divergent branch);
+//     the plans are made on the device (plan.hip);
 //   * each thread keeps its slots (packed local column + segment id, and
 //     the value) in registers for all fields of the launch;
 //   * per field, the tile's distinct columns of x are loaded once (ascending
@@ -31,7 +32,6 @@
 // true division by the area.
 #include <algorithm>
 #include <type_traits>
-#include <vector>
 
 #include "regridder.hpp"
 #include "tileplan.hpp"
@@ -267,57 +267,6 @@ __global__ void __launch_bounds__(G::kThreads) k_spmm_tiles(
         // segment sums are done); its segment writes follow its first barrier
         // (all row passes of this iteration are done)
     }
-}
-
-// ---- plan ----------------------------------------------------------------------------
-const TilePlan &tile_plan(const Csr &A, int style, int geom, hipStream_t s) {
-    std::lock_guard<std::mutex> lock(A.plan_mu);
-    if (A.tiles[style][geom]) return *A.tiles[style][geom];
-    auto P = std::make_shared<TilePlan>();
-    const int64_t nrows = A.nrows, nnz = A.nnz;
-    std::vector<int64_t> rp(nrows + 1);
-    std::vector<int32_t> col(nnz);
-    std::vector<double> val(nnz);
-    CR_HIP(hipMemcpyAsync(rp.data(), A.rowptr.p, A.rowptr.bytes(), hipMemcpyDeviceToHost, s));
-    if (nnz) {
-        CR_HIP(hipMemcpyAsync(col.data(), A.col.p, A.col.bytes(), hipMemcpyDeviceToHost, s));
-        CR_HIP(hipMemcpyAsync(val.data(), A.val.p, A.val.bytes(), hipMemcpyDeviceToHost, s));
-    }
-    CR_HIP(hipStreamSynchronize(s));
-    HostTilePlan H = geom == kGeomL
-                         ? make_host_tile_plan<GeomL>(style, nrows, A.ncols, rp.data(), col.data(), val.data())
-                         : make_host_tile_plan<GeomS>(style, nrows, A.ncols, rp.data(), col.data(), val.data());
-    P->geom = geom;
-
-    P->ntiles = (int64_t)H.desc.size();
-    P->nslots = (int64_t)H.key.size();
-    P->nhuge = (int64_t)H.huge.size();
-    P->tiles.alloc_kept(H.desc.size(), s);
-    P->key.alloc_kept(H.key.size(), s);
-    P->val32.alloc_kept(H.v32.size(), s);
-    P->val64.alloc_kept(H.v64.size(), s);
-    P->cols.alloc_kept(H.cols.size(), s);
-    P->rseg.alloc_kept(H.rseg.size(), s);
-    P->huge.alloc_kept(H.huge.size(), s);
-    auto up = [&](void *dst, const void *src, size_t bytes) {
-        if (bytes) CR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
-    };
-    up(P->tiles.p, H.desc.data(), H.desc.size() * sizeof(TileDesc));
-    up(P->key.p, H.key.data(), H.key.size() * sizeof(uint32_t));
-    up(P->val32.p, H.v32.data(), H.v32.size() * sizeof(float));
-    up(P->val64.p, H.v64.data(), H.v64.size() * sizeof(double));
-    up(P->cols.p, H.cols.data(), H.cols.size() * sizeof(int32_t));
-    up(P->rseg.p, H.rseg.data(), H.rseg.size() * sizeof(uint16_t));
-    up(P->huge.p, H.huge.data(), H.huge.size() * sizeof(int32_t));
-    CR_HIP(hipStreamSynchronize(s));
-    A.tiles[style][geom] = P;
-    return *A.tiles[style][geom];
-}
-
-void drop_tile_plan(const Csr &A) {
-    std::lock_guard<std::mutex> lock(A.plan_mu);
-    for (auto &byStyle : A.tiles)
-        for (auto &p : byStyle) p.reset();
 }
 
 // ---- launch --------------------------------------------------------------------------

@@ -11,6 +11,8 @@ for the file:line map):
 * :func:`transpose`  -- the CSR of ``transpose(R).intersections`` (regridder.jl:49-50)
 * :func:`clip_area`, :func:`polygon_area`, :func:`cell_areas` -- the per-pair
   operator (regridder.jl:96-103) and ``areas`` (regridder.jl:165-178)
+* :func:`tile_plan`  -- the library's own staged-apply layout (cr_tileplan.cpp;
+  no reference counterpart), the checker of the device planner
 * :func:`dunavant`, :func:`gll`, :func:`lagrange`, :func:`element_map`,
   :func:`inverse_element_map`, :func:`element_jacobian`, :func:`se_moments` --
   the spectral-element weights (cr_oracle_se.c; triangle_quadrature.jl and
@@ -91,6 +93,13 @@ def lib():
     L.orc_se_moments.argtypes = [_dp, _dp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp, _dp]
     L.orc_transpose.restype = None
     L.orc_transpose.argtypes = [_i64p, _i32p, _dp, ctypes.c_int64, ctypes.c_int64, _i64p, _i32p, _dp]
+    L.oracle_tile_plan_make.restype = ctypes.c_void_p
+    L.oracle_tile_plan_make.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64,
+                                        _i64p, _i32p, _dp, _i64p]
+    L.oracle_tile_plan_copy.restype = None
+    L.oracle_tile_plan_copy.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 7
+    L.oracle_tile_plan_free.restype = None
+    L.oracle_tile_plan_free.argtypes = [ctypes.c_void_p]
     _lib = L
     return L
 
@@ -299,6 +308,41 @@ def transpose(rowptr, col, val, nrows: int, ncols: int):
 
 
 # ------------------------------------------------------ spectral elements --
+# Tile descriptor of a plan (72 bytes, the library's layout).
+TILE_DESC = np.dtype([("row0", "<i8"), ("ent_off", "<i8"), ("col_off", "<i8"), ("nrows", "<i4"),
+                      ("ncols", "<i4"), ("ent", "<i4"), ("nseg", "<i4"), ("flush", "<u4", (8,))])
+# Tile geometries: threads, rows, columns, segments per tile.
+TILE_GEOMS = ((256, 512, 1024, 1024), (512, 1024, 2048, 1536))
+
+
+def empty_tile_plan(sizes) -> dict:
+    """Host arrays for a plan of the given sizes (tiles, slots, column-list
+    entries, row-map entries, huge rows)."""
+    nt, ns, nc, nr, nh = (int(x) for x in sizes)
+    return {"desc": np.zeros(nt, TILE_DESC), "key": np.zeros(ns, np.uint32), "v32": np.zeros(ns, np.float32),
+            "v64": np.zeros(ns, np.float64), "cols": np.zeros(nc, np.int32), "rseg": np.zeros(nr, np.uint16),
+            "huge": np.zeros(nh, np.int32)}
+
+
+def tile_plan(style: int, geom: int, rowptr, col, val, ncols: int) -> dict:
+    """The staged-apply plan of a CSR matrix (style 0: rounds, 1: even runs;
+    geom 0/1: TILE_GEOMS) as host arrays."""
+    L = lib()
+    rp = np.ascontiguousarray(rowptr, dtype=np.int64)
+    c = np.ascontiguousarray(col, dtype=np.int32)
+    v = np.ascontiguousarray(val, dtype=np.float64)
+    sizes = np.zeros(5, np.int64)
+    h = L.oracle_tile_plan_make(int(style), int(geom), rp.shape[0] - 1, int(ncols), rp.ctypes.data_as(_i64p),
+                                c.ctypes.data_as(_i32p), v.ctypes.data_as(_dp), sizes.ctypes.data_as(_i64p))
+    try:
+        P = empty_tile_plan(sizes)
+        L.oracle_tile_plan_copy(h, *(P[k].ctypes.data for k in ("desc", "key", "v32", "v64", "cols", "rseg",
+                                                                 "huge")))
+    finally:
+        L.oracle_tile_plan_free(h)
+    return P
+
+
 def se_node_weights(elements, nq: int) -> np.ndarray:
     """W = w_i w_j J(xi_i, eta_j) on the unit sphere, node order e*nq^2 + i + j*nq."""
     x, w = gll(nq)
