@@ -332,10 +332,14 @@ __global__ void __launch_bounds__(G::kThreads) k_plan_fill(const TileRec *__rest
                                                            uint16_t *__restrict__ rseg, int *__restrict__ err) {
     constexpr int NT = G::kThreads, RPT = G::kRows / NT, IPT = G::kEnt, ZMAX = NT * IPT;
     using Sort = hipcub::BlockRadixSort<uint32_t, NT, IPT, uint16_t>;
-    __shared__ typename Sort::TempStorage sort_tmp;
+    // the sort's storage is free once every thread is past the barrier after it
+    __shared__ union {
+        typename Sort::TempStorage sort;
+        uint16_t lc[ZMAX];  // local column of each non-zero
+    } u;
+    uint16_t *lc = u.lc;
     __shared__ int32_t qs[G::kRows + 1];  // first tile-local non-zero of each row
     __shared__ int64_t rk[G::kRows];      // first non-zero of each row in A
-    __shared__ uint16_t lc[ZMAX];         // local column of each non-zero
     __shared__ uint32_t scan_lds[8 * NT / 64];
     __shared__ uint32_t lastkey[NT];
     const TileRec R = recs[blockIdx.x];
@@ -402,7 +406,7 @@ __global__ void __launch_bounds__(G::kThreads) k_plan_fill(const TileRec *__rest
             key[i] = (uint32_t)col[rk[lr] + (q - qs[lr])];
         }
     }
-    Sort(sort_tmp).Sort(key, qv, 0, col_bits);
+    Sort(u.sort).Sort(key, qv, 0, col_bits);
     lastkey[tid] = key[IPT - 1];
     __syncthreads();
     uint32_t fl[1] = {0u}, ftot[1];
@@ -444,7 +448,7 @@ __global__ void __launch_bounds__(G::kThreads) k_plan_fill(const TileRec *__rest
         constexpr int PPT = (G::kSegs + NT - 1) / NT;
         __shared__ uint8_t plen[G::kSegs];
         __shared__ uint16_t pq0[G::kSegs];
-        __shared__ int64_t pk0[G::kSegs];
+        __shared__ uint16_t prow[G::kSegs];  // local row of each piece
         __shared__ uint16_t order[G::kSegs];
         const int nseg = (int)tot[1];
         {
@@ -458,7 +462,7 @@ __global__ void __launch_bounds__(G::kThreads) k_plan_fill(const TileRec *__rest
                         const int len = piece_len(rn[i], j);
                         plen[pc + j] = (uint8_t)len;
                         pq0[pc + j] = (uint16_t)(q + off);
-                        pk0[pc + j] = rk[lr] + off;
+                        prow[pc + j] = (uint16_t)lr;
                         off += len;
                     }
                 }
@@ -524,7 +528,7 @@ __global__ void __launch_bounds__(G::kThreads) k_plan_fill(const TileRec *__rest
             const int pos = k * NT + tid;
             if (pos < nseg) {
                 const int sid = order[pos], len = plen[sid], q0 = pq0[sid];
-                const int64_t kk = pk0[sid];
+                const int64_t kk = rk[prow[sid]] + (q0 - qs[prow[sid]]);
                 for (int e = 0; e < L; ++e) {
                     const int64_t slot = R.ent_off + (int64_t)(off + e) * NT + tid;
                     uint32_t kv = (uint32_t)sid << 16;

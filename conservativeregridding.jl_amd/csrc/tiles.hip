@@ -152,7 +152,9 @@ __global__ void __launch_bounds__(G::kThreads) k_spmm_tiles(
     // bit e: slot e ends a segment (for every lane of the wave when UNI)
     uint32_t flush = 0u;
     if constexpr (UNI) {
-        flush = __builtin_amdgcn_readfirstlane(d.flush[tid >> 6]);
+        // read from the descriptor in memory: indexing the local copy by wave
+        // would put the copy in scratch
+        flush = __builtin_amdgcn_readfirstlane(tiles[tile].flush[tid >> 6]);
     } else {
 #pragma unroll
         for (int e = 0; e < G::kEnt; ++e) {
