@@ -337,7 +337,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 fields, f64 A",
+        # variants 5-7, 18-19 sum in FP32 (Kahan); 20-21 use A rounded to FP32
+        "dtype": ("f32 fields, f32 A, f32 Kahan sums" if variant in (5, 6, 7, 18, 19) else
+                  "f32 fields, f32-rounded A, f64 sums" if variant in (20, 21) else
+                  "f32 fields, f64 A, f64 sums"),
         "data": "synthetic",
         "config": {
             "workload": "0.25deg lon-lat 1440x720 -> HEALPix nside 256 nested (16-gon cells); "

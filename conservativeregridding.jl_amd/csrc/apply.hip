@@ -256,17 +256,17 @@ static void launch_rows(const Csr &A, const T *x, int64_t sc, int64_t sf, T *y, 
 // Staged tile kernel (tiles.hip) followed by the rows it leaves to the long-row
 // kernel.  scale: 1/area for FP32 fields, area for FP64 fields, or nullptr.
 template <class T>
-void launch_tiles(const TilePlan &P, int F, bool acc64, int nchunks, int chunk_major, const T *x, int64_t sc,
+void launch_tiles(const TilePlan &P, int F, int acc, int nchunks, int chunk_major, const T *x, int64_t sc,
                   int64_t sf, T *y, int64_t dc, int64_t df, int64_t nf, const double *scale, int64_t nrows,
                   int64_t ncols, hipStream_t s);
 
 template <class T>
-static void launch_tiled(const Csr &A, int geom, int F, bool acc64, int nchunks, int chunk_major, const T *x,
+static void launch_tiled(const Csr &A, int geom, int F, int acc, int nchunks, int chunk_major, const T *x,
                          int64_t sc, int64_t sf, T *y, int64_t dc, int64_t df, int64_t nf, const double *area,
                          const double *inv, hipStream_t s) {
     constexpr bool f32 = sizeof(T) == 4;
     const TilePlan &P = tile_plan(A, f32 ? kRounds : kEven, geom, s);
-    launch_tiles<T>(P, F, acc64, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, f32 ? inv : area, A.nrows,
+    launch_tiles<T>(P, F, acc, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, f32 ? inv : area, A.nrows,
                     A.ncols, s);
     if (P.nhuge) {
         constexpr int FB = f32 ? 8 : 1;
@@ -283,7 +283,7 @@ static void launch_tiled(const Csr &A, int geom, int F, bool acc64, int nchunks,
 struct F32Variant {
     bool tiled;
     int width;        // FB (row kernel) or F (tile kernel)
-    bool acc64;       // tile kernel: FP64 accumulation
+    int acc;          // tile kernel: 0 Kahan FP32 sums, 1 FP64 sums, 2 FP64 sums of FP32 values
     int chunks;       // tile kernel: field chunks per launch (0: enough workgroups)
     int chunk_major;  // tile kernel: sweep one field chunk at a time
     int geom;         // tile kernel: TileGeomId
@@ -294,15 +294,17 @@ constexpr F32Variant kF32Variants[] = {
     {true, 4, false, 0, 0, 0},   {true, 8, false, 0, 0, 0},   {true, 2, true, 0, 0, 0},
     {true, 4, true, 0, 0, 0},    {true, 8, true, 0, 0, 0},    {true, 4, true, 4, 0, 0},
     {true, 4, true, 8, 1, 0},    {true, 4, true, 16, 1, 0},   {true, 4, true, 1, 0, 0},
-    {true, 4, true, 0, 0, 1},    {true, 2, true, 0, 0, 1},    {true, 4, true, 4, 0, 1}};
+    {true, 4, true, 0, 0, 1},    {true, 2, true, 0, 0, 1},    {true, 4, true, 4, 0, 1},
+    {true, 4, false, 0, 0, 1},   {true, 4, false, 4, 0, 1},   {true, 4, 2, 0, 0, 1},
+    {true, 4, 2, 4, 0, 1}};
 constexpr int kNumVariants = sizeof(kF32Variants) / sizeof(kF32Variants[0]);
-constexpr int kDefaultF32Variant = 9;  // tiles, 4 fields per stage, FP64 sums
+constexpr int kDefaultF32Variant = 21;  // 1024-row tiles, 4 fields per stage, FP64 sums of FP32 A
 
 static void launch_f32_variant(int v, const Csr &A, const float *x, int64_t sc, int64_t sf, float *y,
                                int64_t dc, int64_t df, int64_t nf, const double *inv, hipStream_t s) {
     const F32Variant &V = kF32Variants[v];
     if (V.tiled) {
-        launch_tiled<float>(A, V.geom, V.width, V.acc64, V.chunks, V.chunk_major, x, sc, sf, y, dc, df, nf,
+        launch_tiled<float>(A, V.geom, V.width, V.acc, V.chunks, V.chunk_major, x, sc, sf, y, dc, df, nf,
                             nullptr, inv, s);
         return;
     }
@@ -321,7 +323,7 @@ constexpr int kDefaultF64Variant = 1;
 // Kernel choice: FP64 -> the tile kernel (1 or 2 fields per stage) unless the
 // row kernel is pinned; FP32 with unit field strides and many fields ->
 // fields-fastest kernel; other FP32 layouts -> the autotuned (or pinned, or
-// default: tiles, 4 fields per stage) variant.
+// default: 1024-row tiles, 4 fields per stage, FP64 sums) variant.
 void apply(const cr_regridder *R, bool transpose, int dtype, int64_t nf, const void *src,
            int64_t sc, int64_t sf, void *dst, int64_t dc, int64_t df, bool normalize, hipStream_t s,
            int force_variant = -1) {

@@ -93,18 +93,32 @@ struct alignas(sizeof(T) * F >= 16 ? 16 : sizeof(T) * F) Pack {
 };
 
 // MODE 0: T accumulation (FP64 fields: FMA; FP32 fields: Kahan);
-// MODE 1: FP32 fields accumulated in FP64 (values from the FP64 A).
+// MODE 1: FP32 fields accumulated in FP64 (values from the FP64 A);
+// MODE 2: as MODE 1 with the values rounded to FP32.
 // OFF32: every cell offset (cell * stride) fits 32 bits, so loads and stores
 // use a per-field base address plus a 32-bit offset.
+// val: the register type of A's values, load: the type they are read as.
 template <int MODE, class T>
 struct TileAcc {
     using type = SegAcc<T>;
     using val = T;
+    using load = T;
 };
 template <>
 struct TileAcc<1, float> {
     using type = SegAcc64;
     using val = double;
+    using load = double;
+};
+// MODE 2: FP32 fields accumulated in FP64 from A rounded to FP32 (8 bytes per
+// slot instead of 12; A's rounding, 2^-24 relative, is below the FP32 result's).
+// The values stay FP32 in registers and are widened at each product (widening
+// them once per launch measured slower: twice the registers per slot).
+template <>
+struct TileAcc<2, float> {
+    using type = SegAcc64;
+    using val = float;
+    using load = float;
 };
 
 // UNI: segment ends are the plan's wave-uniform masks (kRounds plans); else
@@ -113,7 +127,7 @@ template <class G, class T, int F, int MODE, bool OFF32, bool UNI>
 __global__ void __launch_bounds__(G::kThreads) k_spmm_tiles(
     const TileDesc *__restrict__ tiles, int64_t ntiles, int64_t tiles_per_xcd, int nchunks,
     int64_t chunk, int chunk_major, const uint32_t *__restrict__ keys,
-    const typename TileAcc<MODE, T>::val *__restrict__ vals,
+    const typename TileAcc<MODE, T>::load *__restrict__ vals,
     const int32_t *__restrict__ cols, const uint16_t *__restrict__ rseg, const T *__restrict__ x,
     int64_t sc, int64_t sf, T *__restrict__ y, int64_t dc, int64_t df, int64_t nf,
     const double *__restrict__ scale) {
@@ -146,7 +160,7 @@ __global__ void __launch_bounds__(G::kThreads) k_spmm_tiles(
         if (e < E) {
             const int64_t q = d.ent_off + (int64_t)e * G::kThreads + tid;
             key[e] = keys[q];
-            v[e] = vals[q];
+            v[e] = (V)vals[q];
         }
     }
     // bit e: slot e ends a segment (for every lane of the wave when UNI)
@@ -282,7 +296,7 @@ static void launch_f(const TilePlan &P, int nchunks, int chunk_major, const T *x
     nchunks = (int)((nf + chunk - 1) / chunk);
     const int64_t blocks = tiles_per_xcd * 8 * nchunks;
     CR_REQUIRE(blocks < (int64_t)1 << 31, CR_UNSUPPORTED, "too many tiles x field chunks for one launch");
-    using V = typename TileAcc<MODE, T>::val;
+    using V = typename TileAcc<MODE, T>::load;
     const V *vals;
     if constexpr (sizeof(V) == 4) vals = (const V *)P.val32.p;
     else vals = (const V *)P.val64.p;
@@ -308,24 +322,28 @@ int default_chunks(const TilePlan &P, int64_t nf, int F) {
     return (int)c;
 }
 
-// F: fields per LDS stage; acc64: FP32 fields accumulated in FP64 (else Kahan).
+// F: fields per LDS stage; acc: FP32 fields summed with Kahan (0), in FP64 from
+// the FP64 A (1), in FP64 from A rounded to FP32 (2, 1024-row tiles only).
 // FP32 fields use kRounds plans (wave-uniform segment ends), FP64 fields kEven
 // plans (no padding; the same segments for any field count, so a batched FP64
 // apply equals the loop of single-field applies bit for bit).
 template <class T>
-void launch_tiles(const TilePlan &P, int F, bool acc64, int nchunks, int chunk_major, const T *x, int64_t sc,
+void launch_tiles(const TilePlan &P, int F, int acc, int nchunks, int chunk_major, const T *x, int64_t sc,
                   int64_t sf, T *y, int64_t dc, int64_t df, int64_t nf, const double *scale, int64_t nrows,
                   int64_t ncols, hipStream_t s) {
     if (P.ntiles == 0) return;
+    const bool acc64 = acc != 0;
     if (nchunks <= 0) nchunks = default_chunks(P, nf, F);
     nchunks = (int)std::max<int64_t>(1, std::min<int64_t>(nchunks, (nf + F - 1) / F));
 #define CR_TILE_ARGS P, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s
     if constexpr (sizeof(T) == 4) {
         if (P.geom == kGeomL) {
-            CR_REQUIRE(acc64, CR_UNSUPPORTED, "1024-row tiles are built with FP64 sums only");
-            if (F == 2) launch_f<GeomL, T, 2, 1, true>(CR_TILE_ARGS);
+            if (!acc64) launch_f<GeomL, T, 4, 0, true>(CR_TILE_ARGS);
+            else if (acc == 2) launch_f<GeomL, T, 4, 2, true>(CR_TILE_ARGS);
+            else if (F == 2) launch_f<GeomL, T, 2, 1, true>(CR_TILE_ARGS);
             else launch_f<GeomL, T, 4, 1, true>(CR_TILE_ARGS);
         } else if (acc64) {
+            CR_REQUIRE(acc == 1, CR_UNSUPPORTED, "FP32 values of A are kept for 1024-row tiles only");
             switch (F) {
                 case 2: launch_f<GeomS, T, 2, 1, true>(CR_TILE_ARGS); break;
                 case 8: launch_f<GeomS, T, 8, 1, true>(CR_TILE_ARGS); break;
@@ -350,10 +368,10 @@ void launch_tiles(const TilePlan &P, int F, bool acc64, int nchunks, int chunk_m
 #undef CR_TILE_ARGS
 }
 
-template void launch_tiles<float>(const TilePlan &, int, bool, int, int, const float *, int64_t, int64_t,
+template void launch_tiles<float>(const TilePlan &, int, int, int, int, const float *, int64_t, int64_t,
                                   float *, int64_t, int64_t, int64_t, const double *, int64_t, int64_t,
                                   hipStream_t);
-template void launch_tiles<double>(const TilePlan &, int, bool, int, int, const double *, int64_t, int64_t,
+template void launch_tiles<double>(const TilePlan &, int, int, int, int, const double *, int64_t, int64_t,
                                    double *, int64_t, int64_t, int64_t, const double *, int64_t, int64_t,
                                    hipStream_t);
 

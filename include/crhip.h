@@ -257,11 +257,14 @@ cr_status cr_regrid_tune(cr_regridder *R, int32_t transpose, int64_t nfields, co
  * variant cr_regrid uses for one direction of R (-1 restores the default).
  *   CR_F32: 0-4 row kernel with 1, 2, 4, 8, 16 fields per thread; 5-7 staged
  *           tile kernel with 2, 4, 8 fields per LDS stage and FP32 Kahan sums;
- *           8-10 the tile kernel with 2, 4, 8 fields per stage and FP64 sums
- *           (default 9); 11-14 variant 9 with 4 field chunks, 8 and 16 chunks
+ *           8-10 the tile kernel with 2, 4, 8 fields per stage and FP64 sums;
+ *           11-14 variant 9 with 4 field chunks, 8 and 16 chunks
  *           swept one at a time, and a single chunk; 15-17 FP64 sums on
  *           1024-row tiles (512 threads) with 4, 2 and 4 fields per stage (17:
- *           4 field chunks).  FP32 fields with unit field strides and >= 64
+ *           4 field chunks); 18-19 FP32 Kahan sums on 1024-row tiles with 4
+ *           fields per stage (19: 4 field chunks); 20-21 FP64 sums of A
+ *           rounded to FP32 on 1024-row tiles, 4 fields per stage (21: 4 field
+ *           chunks; the default).  FP32 fields with unit field strides and >= 64
  *           fields always use the fields-fastest kernel.
  *   CR_F64: 0 row kernel, 1 staged tile kernel (default).
  */
