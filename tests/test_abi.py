@@ -58,3 +58,13 @@ def test_last_error_is_thread_local_text(cr):
     st = L.lib.cr_gen_cubed_sphere_points(None, 0)
     assert st == L.CR_INVALID_ARGUMENT
     assert "bad arguments" in L.last_error()
+
+
+def test_tile_plan_entry_points_reject_bad_arguments(cr):
+    """Host-side argument checks of the tile-plan entry points (no device call)."""
+    L = cr._lib
+    sizes = (ctypes.c_int64 * 5)()
+    assert L.lib.cr_regrid_tile_plan(None, 0, 0, 0, sizes, None) == L.CR_INVALID_ARGUMENT
+    assert L.lib.cr_regrid_copy_tile_plan(None, 0, 0, 0, None, None, None, None, None, None, None) \
+        == L.CR_INVALID_ARGUMENT
+    assert "null" in L.last_error()
