@@ -110,7 +110,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the apply would shard "
-              "by fields with no communication, see DESIGN.md)", file=sys.stderr)
+              "by fields, each GPU holding all of A and k/G fields, with no communication)", file=sys.stderr)
         sys.exit(2)
 
     import torch

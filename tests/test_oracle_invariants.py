@@ -1,5 +1,5 @@
 """Whole-grid invariants of the CPU oracle build (what pins it where no
-reference value exists, SURVEY-style: row/col sums vs geometric areas as in
+reference value exists: row/col sums vs geometric areas as in
 lib/ConservativeRegriddingTestHelpers/src/ConservativeRegriddingTestHelpers.jl:70-73,
 conservation as in test/usecases/*.jl, the candidate superset of
 test/trees/quadtree_cursors.jl:55-81)."""
