@@ -48,6 +48,7 @@ struct VertexSink {
     double *g_subj, *g_clip;           // caller-vertex gradients (may be null)
     double *lacc_subj, *lacc_clip;     // LDS accumulators of one side's slots, or null
     double *own_clip;                  // the pair's own 4 x 3 clip-slot sums in LDS, or null
+    double *own_subj;                  // the pair's own 4 x 3 subject-slot sums in LDS, or null
 
     __device__ d3 sv(int e) const { return load3(subj + 3 * (e % ns)); }
     __device__ d3 cv(int k) const {
@@ -67,8 +68,16 @@ struct VertexSink {
     }
     __device__ void add_subj(int e, const d3 &g) const {
         e %= ns;
-        if (lacc_subj) add_local(lacc_subj, e, tangential(g, sv(e)));
-        else add(g_subj, subj_vi[e], tangential(g, sv(e)));
+        if (lacc_subj) {
+            add_local(lacc_subj, e, tangential(g, sv(e)));
+        } else if (own_subj) {  // this pair's own sums (no other thread touches them)
+            const d3 t = tangential(g, sv(e));
+            own_subj[3 * e] += t.x;
+            own_subj[3 * e + 1] += t.y;
+            own_subj[3 * e + 2] += t.z;
+        } else {
+            add(g_subj, subj_vi[e], tangential(g, sv(e)));
+        }
     }
     __device__ void add_clip(int k, const d3 &g) const {
         int kk = k % 4;
@@ -107,12 +116,16 @@ struct VertexSink {
 // One wave per 64 consecutive non-zeros.  CSR rows are destination cells, so a
 // wave's pairs share few destination cells (about 12 on the benchmark): their
 // slots' gradients are summed in LDS (rows r0 .. r0 + kGradRows - 1 of the
-// wave) and reach memory once per wave.  A source cell that clips has its 4
-// slots summed per pair and written once per pair; other contributions use
-// global FP64 atomics directly.
+// wave) and reach memory once per wave.  A source cell of at most 4 vertices
+// (clipping or clipped) has its slots summed per pair and written once per
+// pair; other contributions use global FP64 atomics directly.
 constexpr int kGradRows = 16;
 
-__global__ void __launch_bounds__(64) k_clip_grad(
+// Two waves per SIMD: the compiler then keeps 256 registers and spills ~70 to
+// scratch (the kernel's 300 registers allow one wave), and the second wave
+// hides the atomics' latency (9.9 against 12.1 ms on the benchmark; three or
+// four waves spill more and run slower).
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_clip_grad(
     const int32_t *__restrict__ rowid, const int32_t *__restrict__ col, const double *__restrict__ coef,
     int64_t nnz, const double *__restrict__ dst_cells, int dst_nv, const int32_t *__restrict__ dst_vi,
     const double *__restrict__ src_cells, int src_nv, const int32_t *__restrict__ src_vi,
@@ -135,16 +148,22 @@ __global__ void __launch_bounds__(64) k_clip_grad(
         double *mine = (g_dst && d - r0 < kGradRows) ? lacc + (d - r0) * dst_nv * 3 : nullptr;
         VertexSink sink;
         if (clip_is_dst) {
+            // a source subject of at most 4 vertices sums its slots per pair
             sink = VertexSink{sp, dp, src_nv, dst_nv, src_vi + (int64_t)src_nv * s, dst_vi + (int64_t)dst_nv * d,
-                              g_src, g_dst, nullptr, mine, nullptr};
+                              g_src, g_dst, nullptr, mine, nullptr, (g_src && src_nv <= 4) ? my_own : nullptr};
         } else {
             sink = VertexSink{dp, sp, dst_nv, src_nv, dst_vi + (int64_t)dst_nv * d, src_vi + (int64_t)src_nv * s,
-                              g_dst, g_src, mine, nullptr, g_src ? my_own : nullptr};
+                              g_dst, g_src, mine, nullptr, g_src ? my_own : nullptr, nullptr};
         }
         clip_area_backward<4, VertexSink>(sink.clip, sink.nc, sink.subj, sink.ns, c, sink);
-        if (sink.own_clip)
-            for (int kk = 0; kk < src_nv && kk < 4; ++kk)
-                VertexSink::add(g_src, sink.clip_vi[kk], mk3(my_own[3 * kk], my_own[3 * kk + 1], my_own[3 * kk + 2]));
+        // the per-pair sums of the source cell's 4 slots, one atomic per component
+        if (sink.own_clip || sink.own_subj) {
+            const int32_t *vi = sink.own_clip ? sink.clip_vi : sink.subj_vi;
+            for (int kk = 0; kk < src_nv && kk < 4; ++kk) {
+                const d3 v = mk3(my_own[3 * kk], my_own[3 * kk + 1], my_own[3 * kk + 2]);
+                if (v.x != 0.0 || v.y != 0.0 || v.z != 0.0) VertexSink::add(g_src, vi[kk], v);
+            }
+        }
     }
     __syncthreads();
     if (!g_dst) return;
