@@ -100,6 +100,151 @@ def read_traffic(name, variant):
     return None
 
 
+def _time_ms(torch, fn, reps):
+    """Mean device time of fn() over reps calls (HIP events on the current stream)."""
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def _build_ms(cr, torch, dst, src, reps=3, **kw):
+    """A regridder and the median wall time of `reps` further device builds."""
+    R = cr.Regridder(dst, src, **kw)
+    ts = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        Rb = cr.Regridder(dst, src, **kw)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t)
+        del Rb
+    return R, float(np.median(ts)) * 1e3
+
+
+def _jittered_cubed_sphere(cr, ne, seed, frac=0.05):
+    """Cubed-sphere points with every face-interior vertex moved in its tangent
+    plane by N(0, frac * cell width), renormalised (cells stay convex)."""
+    P = cr.cubed_sphere_points(ne).copy()
+    rng = np.random.default_rng(seed)
+    inner = P[:, 1:-1, 1:-1, :]
+    noise = rng.normal(0.0, frac * (np.pi / 2 / ne), size=inner.shape)
+    noise -= np.sum(noise * inner, axis=-1, keepdims=True) * inner
+    moved = inner + noise
+    P[:, 1:-1, 1:-1, :] = moved / np.linalg.norm(moved, axis=-1, keepdims=True)
+    return P
+
+
+def report_configs(cr, torch, reps):
+    """The reference's other configurations, measured on one GPU (one JSON line
+    each): build pairs/s, single-field FP64 applies forward and transposed, the
+    reverse mode, and the largest configuration's 64-field FP32 batched apply
+    forward and transposed (plain launches and a graph replay)."""
+    out = []
+
+    def single_field(name, R, nd, ns):
+        x = torch.rand(ns, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(0))
+        y = torch.zeros(nd, dtype=torch.float64, device="cuda")
+        RT = cr.transpose(R)
+        # prepared launches: the device time of back-to-back applies, not the Python call
+        fwd = _time_ms(torch, cr.regrid_plan(y, R, x), reps)
+        bwd = _time_ms(torch, cr.regrid_plan(x, RT, y), reps)
+        # conservation of the forward apply: sum(y a_d) = sum(x a_s) when the destination is covered
+        a_d = torch.tensor(R.dst_areas, device="cuda")
+        a_s = torch.tensor(R.src_areas, device="cuda")
+        cr.regrid_(y, R, x)
+        drift = abs(float((y * a_d).sum() / a_d.sum() - (x * a_s).sum() / a_s.sum()))
+        b = apply_bytes(R.nnz, nd, ns, 1, 8), apply_bytes(R.nnz, ns, nd, 1, 8)
+        log(f"[configs] {name}: FP64 single field forward {fwd * 1e3:.1f} us ({b[0] / (fwd * 1e-3) / 1e9:.0f} GB/s), "
+            f"transpose {bwd * 1e3:.1f} us ({b[1] / (bwd * 1e-3) / 1e9:.0f} GB/s), mean drift {drift:.1e}")
+        return {"fp64_forward_us": round(fwd * 1e3, 2), "fp64_forward_GBps": round(b[0] / (fwd * 1e-3) / 1e9, 1),
+                "fp64_transpose_us": round(bwd * 1e3, 2), "fp64_transpose_GBps": round(b[1] / (bwd * 1e-3) / 1e9, 1),
+                "fp64_mean_drift": drift}
+
+    # 4 deg -> 5 deg lon-lat, normalize=true
+    src = cr.Grid.from_points(cr.lonlat_points(90, 45))
+    dst = cr.Grid.from_points(cr.lonlat_points(72, 36))
+    R, ms = _build_ms(cr, torch, dst, src, normalize=True)
+    log(f"[configs] 4deg -> 5deg lon-lat: build {ms:.2f} ms, nnz {R.nnz} ({R.nnz / (ms * 1e-3) / 1e6:.1f} M pairs/s)")
+    rec = {"workload": "4deg lon-lat 90x45 -> 5deg lon-lat 72x36, normalize=true", "nnz": R.nnz,
+           "build_ms": round(ms, 3), "build_pairs_per_s": round(R.nnz / (ms * 1e-3), 1)}
+    rec.update(single_field("4deg -> 5deg", R, 72 * 36, 90 * 45))
+    out.append(rec)
+
+    # 1 deg lon-lat -> C48
+    src = cr.Grid.from_points(cr.lonlat_points(360, 180))
+    dst = cr.Grid.from_points(cr.cubed_sphere_points(48))
+    R, ms = _build_ms(cr, torch, dst, src)
+    log(f"[configs] 1deg lon-lat -> C48: build {ms:.2f} ms, nnz {R.nnz} ({R.nnz / (ms * 1e-3) / 1e6:.1f} M pairs/s)")
+    rec = {"workload": "1deg lon-lat 360x180 -> cubed sphere C48", "nnz": R.nnz, "build_ms": round(ms, 3),
+           "build_pairs_per_s": round(R.nnz / (ms * 1e-3), 1)}
+    rec.update(single_field("1deg -> C48", R, 6 * 48 * 48, 360 * 180))
+    out.append(rec)
+
+    # jittered C96 -> 1 deg lon-lat, reverse mode
+    src = cr.Grid.from_points(_jittered_cubed_sphere(cr, 96, seed=4))
+    dst = cr.Grid.from_points(cr.lonlat_points(360, 180))
+    R, ms = _build_ms(cr, torch, dst, src)
+    ns, nd = 6 * 96 * 96, 360 * 180
+    x = torch.rand(ns, dtype=torch.float64, device="cuda")
+    y = cr.regrid(R, x)
+    gy = torch.ones_like(y)
+    ms_bwd = _time_ms(torch, lambda: cr.regrid_backward(R, dst, src, x, y, gy), reps)
+    log(f"[configs] jittered C96 -> 1deg: build {ms:.2f} ms, nnz {R.nnz}; backward (field + vertex gradients) "
+        f"{ms_bwd:.3f} ms ({R.nnz / (ms_bwd * 1e-3) / 1e6:.0f} M pairs/s)")
+    out.append({"workload": "jittered cubed sphere C96 -> 1deg lon-lat, reverse mode", "nnz": R.nnz,
+                "build_ms": round(ms, 3), "build_pairs_per_s": round(R.nnz / (ms * 1e-3), 1),
+                "backward_ms": round(ms_bwd, 4), "backward_pairs_per_s": round(R.nnz / (ms_bwd * 1e-3), 1)})
+    del R, x, y, gy
+
+    # HEALPix nside 1024 -> 0.1 deg lon-lat, 64 FP32 fields forward and transposed
+    src = cr.Grid.healpix(1024, "nested", step=1)
+    dst = cr.Grid.from_points(cr.lonlat_points(3600, 1800))
+    R, ms = _build_ms(cr, torch, dst, src, reps=1)
+    ns, nd, k = src.ncells, dst.ncells, 64
+    log(f"[configs] HEALPix 1024 -> 0.1deg: build {ms:.1f} ms, nnz {R.nnz} ({R.nnz / (ms * 1e-3) / 1e6:.0f} M pairs/s)")
+    g = torch.Generator("cuda").manual_seed(0)
+    X = torch.rand(k, ns, dtype=torch.float32, device="cuda", generator=g)
+    Y = torch.zeros(k, nd, dtype=torch.float32, device="cuda")
+    Xc, Yc = X.t(), Y.t()  # cells fastest: (n, 64) with cell stride 1
+    RT = cr.transpose(R)
+    f_fwd = cr.regrid_plan(Yc, R, Xc)
+    f_bwd = cr.regrid_plan(Xc, RT, Yc)
+    ms_f = _time_ms(torch, f_fwd, reps)
+    ms_t = _time_ms(torch, f_bwd, reps)
+    graph = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        f_fwd()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(graph):
+        f_fwd()
+    ms_g = _time_ms(torch, graph.replay, reps)
+    bf, bt = apply_bytes(R.nnz, nd, ns, k, 4), apply_bytes(R.nnz, ns, nd, k, 4)
+    x0 = X[0].double().contiguous()
+    y0 = cr.regrid(R, x0)
+    gy0 = torch.ones_like(y0)
+    ms_bwd = _time_ms(torch, lambda: cr.regrid_backward(R, dst, src, x0, y0, gy0), max(1, reps // 2))
+    log(f"[configs] HEALPix 1024 -> 0.1deg, 64 FP32 fields: forward {ms_f:.3f} ms ({bf / (ms_f * 1e-3) / 1e9:.0f} GB/s), "
+        f"graph replay {ms_g:.3f} ms, transpose {ms_t:.3f} ms ({bt / (ms_t * 1e-3) / 1e9:.0f} GB/s); "
+        f"backward (one FP64 field, vertex gradients) {ms_bwd:.2f} ms")
+    out.append({"workload": "HEALPix nside 1024 nested -> 0.1deg lon-lat 3600x1800; 64 FP32 fields", "nnz": R.nnz,
+                "build_ms": round(ms, 3), "build_pairs_per_s": round(R.nnz / (ms * 1e-3), 1),
+                "fp32_forward_ms": round(ms_f, 4), "fp32_forward_GBps": round(bf / (ms_f * 1e-3) / 1e9, 1),
+                "fp32_forward_graph_ms": round(ms_g, 4),
+                "fp32_transpose_ms": round(ms_t, 4), "fp32_transpose_GBps": round(bt / (ms_t * 1e-3) / 1e9, 1),
+                "backward_ms": round(ms_bwd, 3)})
+    for rec in out:
+        print(json.dumps({"configs_report": rec}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +252,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline leg")
     ap.add_argument("--builds", type=int, default=3, help="timed device builds")
+    ap.add_argument("--configs", action="store_true",
+                    help="measure the other reference configurations instead of the headline step")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the apply would shard "
@@ -119,6 +266,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.cuda.get_device_name(0)
     log(f"[bench] device {dev}; library {cr.version()}")
+    if args.configs:
+        report_configs(cr, torch, max(10, args.steps))
+        return
 
     NLON, NLAT, NSIDE, STEP, K = 1440, 720, 256, 4, 256
     t0 = time.time()
