@@ -215,9 +215,14 @@ def report_configs(cr, torch, reps):
     RT = cr.transpose(R)
     f_fwd = cr.regrid_plan(Yc, R, Xc)
     f_bwd = cr.regrid_plan(Xc, RT, Yc)
-    ms_f = _time_ms(torch, f_fwd, reps)
+    ms_f = _time_ms(torch, f_fwd, reps)  # the default variant
     ms_t = _time_ms(torch, f_bwd, reps)
-    graph = torch.cuda.CUDAGraph()
+    # the autotuner's pick per direction (block-ordered plans suit the forward's row strips)
+    v_f, _ = cr.regrid_tune(Yc, R, Xc)
+    v_t, _ = cr.regrid_tune(Xc, RT, Yc)
+    ms_ft = _time_ms(torch, f_fwd, reps)  # the prepared launches now run the tuned variants
+    ms_tt = _time_ms(torch, f_bwd, reps)
+    graph = torch.cuda.CUDAGraph()  # the tuned forward captured in a HIP graph
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(side):
@@ -232,14 +237,19 @@ def report_configs(cr, torch, reps):
     y0 = cr.regrid(R, x0)
     gy0 = torch.ones_like(y0)
     ms_bwd = _time_ms(torch, lambda: cr.regrid_backward(R, dst, src, x0, y0, gy0), max(1, reps // 2))
-    log(f"[configs] HEALPix 1024 -> 0.1deg, 64 FP32 fields: forward {ms_f:.3f} ms ({bf / (ms_f * 1e-3) / 1e9:.0f} GB/s), "
-        f"graph replay {ms_g:.3f} ms, transpose {ms_t:.3f} ms ({bt / (ms_t * 1e-3) / 1e9:.0f} GB/s); "
+    log(f"[configs] HEALPix 1024 -> 0.1deg, 64 FP32 fields, default variant: forward {ms_f:.3f} ms "
+        f"({bf / (ms_f * 1e-3) / 1e9:.0f} GB/s), transpose {ms_t:.3f} ms ({bt / (ms_t * 1e-3) / 1e9:.0f} GB/s); "
+        f"autotuned: forward variant {v_f} {ms_ft:.3f} ms ({bf / (ms_ft * 1e-3) / 1e9:.0f} GB/s, graph replay "
+        f"{ms_g:.3f} ms), transpose variant {v_t} {ms_tt:.3f} ms ({bt / (ms_tt * 1e-3) / 1e9:.0f} GB/s); "
         f"backward (one FP64 field, vertex gradients) {ms_bwd:.2f} ms")
     out.append({"workload": "HEALPix nside 1024 nested -> 0.1deg lon-lat 3600x1800; 64 FP32 fields", "nnz": R.nnz,
                 "build_ms": round(ms, 3), "build_pairs_per_s": round(R.nnz / (ms * 1e-3), 1),
                 "fp32_forward_ms": round(ms_f, 4), "fp32_forward_GBps": round(bf / (ms_f * 1e-3) / 1e9, 1),
-                "fp32_forward_graph_ms": round(ms_g, 4),
+                "fp32_tuned_forward_graph_ms": round(ms_g, 4),
                 "fp32_transpose_ms": round(ms_t, 4), "fp32_transpose_GBps": round(bt / (ms_t * 1e-3) / 1e9, 1),
+                "tuned": {"forward_variant": v_f, "forward_ms": round(ms_ft, 4),
+                          "forward_GBps": round(bf / (ms_ft * 1e-3) / 1e9, 1), "transpose_variant": v_t,
+                          "transpose_ms": round(ms_tt, 4), "transpose_GBps": round(bt / (ms_tt * 1e-3) / 1e9, 1)},
                 "backward_ms": round(ms_bwd, 3)})
     for rec in out:
         print(json.dumps({"configs_report": rec}), flush=True)

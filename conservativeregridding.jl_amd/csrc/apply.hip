@@ -263,9 +263,9 @@ void launch_tiles(const TilePlan &P, int F, int acc, int nchunks, int chunk_majo
 template <class T>
 static void launch_tiled(const Csr &A, int geom, int F, int acc, int nchunks, int chunk_major, const T *x,
                          int64_t sc, int64_t sf, T *y, int64_t dc, int64_t df, int64_t nf, const double *area,
-                         const double *inv, hipStream_t s) {
+                         const double *inv, hipStream_t s, bool blocks = false) {
     constexpr bool f32 = sizeof(T) == 4;
-    const TilePlan &P = tile_plan(A, f32 ? kRounds : kEven, geom, s);
+    const TilePlan &P = tile_plan(A, f32 ? kRounds : kEven, geom, s, blocks);
     launch_tiles<T>(P, F, acc, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, f32 ? inv : area, A.nrows,
                     A.ncols, s);
     if (P.nhuge) {
@@ -287,6 +287,7 @@ struct F32Variant {
     int chunks;       // tile kernel: field chunks per launch (0: enough workgroups)
     int chunk_major;  // tile kernel: sweep one field chunk at a time
     int geom;         // tile kernel: TileGeomId
+    int blocks = 0;   // tile kernel: plan over 32-row blocks sorted by their smallest column
 };
 constexpr F32Variant kF32Variants[] = {
     {false, 1, false, 0, 0, 0},  {false, 2, false, 0, 0, 0},  {false, 4, false, 0, 0, 0},
@@ -296,7 +297,7 @@ constexpr F32Variant kF32Variants[] = {
     {true, 4, true, 8, 1, 0},    {true, 4, true, 16, 1, 0},   {true, 4, true, 1, 0, 0},
     {true, 4, true, 0, 0, 1},    {true, 2, true, 0, 0, 1},    {true, 4, true, 4, 0, 1},
     {true, 4, false, 0, 0, 1},   {true, 4, false, 4, 0, 1},   {true, 4, 2, 0, 0, 1},
-    {true, 4, 2, 4, 0, 1}};
+    {true, 4, 2, 4, 0, 1},       {true, 4, 2, 0, 0, 1, 1},    {true, 4, 2, 4, 0, 1, 1}};
 constexpr int kNumVariants = sizeof(kF32Variants) / sizeof(kF32Variants[0]);
 constexpr int kDefaultF32Variant = 21;  // 1024-row tiles, 4 fields per stage, FP64 sums of FP32 A
 
@@ -305,7 +306,7 @@ static void launch_f32_variant(int v, const Csr &A, const float *x, int64_t sc, 
     const F32Variant &V = kF32Variants[v];
     if (V.tiled) {
         launch_tiled<float>(A, V.geom, V.width, V.acc, V.chunks, V.chunk_major, x, sc, sf, y, dc, df, nf,
-                            nullptr, inv, s);
+                            nullptr, inv, s, V.blocks != 0);
         return;
     }
     switch (V.width) {

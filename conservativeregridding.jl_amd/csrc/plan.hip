@@ -646,6 +646,8 @@ struct IsHugeRow {
     __host__ __device__ bool operator()(const int32_t &r) const { return rp[r + 1] - rp[r] > kcols; }
 };
 
+}  // namespace
+
 template <class G, int STYLE>
 void make_plan(const Csr &A, TilePlan &P, hipStream_t s) {
     const int64_t nrows = A.nrows, nnz = A.nnz;
@@ -718,28 +720,36 @@ void make_plan(const Csr &A, TilePlan &P, hipStream_t s) {
     CR_REQUIRE(!bad, CR_INTERNAL, "tile plan: a tile broke its budgets");
 }
 
-}  // namespace
+template void make_plan<GeomS, kRounds>(const Csr &, TilePlan &, hipStream_t);
+template void make_plan<GeomS, kEven>(const Csr &, TilePlan &, hipStream_t);
+template void make_plan<GeomL, kRounds>(const Csr &, TilePlan &, hipStream_t);
+template void make_plan<GeomL, kEven>(const Csr &, TilePlan &, hipStream_t);
 
-const TilePlan &tile_plan(const Csr &A, int style, int geom, hipStream_t s) {
+template <class G, int STYLE>
+void make_block_plan(const Csr &A, TilePlan &P, hipStream_t s);  // planblocks.hip
+
+const TilePlan &tile_plan(const Csr &A, int style, int geom, hipStream_t s, bool blocks) {
     std::lock_guard<std::mutex> lock(A.plan_mu);
-    if (A.tiles[style][geom]) return *A.tiles[style][geom];
+    auto &slot = A.tiles[style][geom][blocks ? 1 : 0];
+    if (slot) return *slot;
     auto P = std::make_shared<TilePlan>();
     P->geom = geom;
     if (geom == kGeomL) {
-        if (style == kEven) make_plan<GeomL, kEven>(A, *P, s);
-        else make_plan<GeomL, kRounds>(A, *P, s);
+        if (style == kEven) blocks ? make_block_plan<GeomL, kEven>(A, *P, s) : make_plan<GeomL, kEven>(A, *P, s);
+        else blocks ? make_block_plan<GeomL, kRounds>(A, *P, s) : make_plan<GeomL, kRounds>(A, *P, s);
     } else {
-        if (style == kEven) make_plan<GeomS, kEven>(A, *P, s);
-        else make_plan<GeomS, kRounds>(A, *P, s);
+        if (style == kEven) blocks ? make_block_plan<GeomS, kEven>(A, *P, s) : make_plan<GeomS, kEven>(A, *P, s);
+        else blocks ? make_block_plan<GeomS, kRounds>(A, *P, s) : make_plan<GeomS, kRounds>(A, *P, s);
     }
-    A.tiles[style][geom] = P;
-    return *A.tiles[style][geom];
+    slot = P;
+    return *slot;
 }
 
 void drop_tile_plan(const Csr &A) {
     std::lock_guard<std::mutex> lock(A.plan_mu);
     for (auto &byStyle : A.tiles)
-        for (auto &p : byStyle) p.reset();
+        for (auto &byGeom : byStyle)
+            for (auto &p : byGeom) p.reset();
 }
 
 }  // namespace cr
@@ -779,7 +789,7 @@ cr_status cr_regrid_copy_tile_plan(const cr_regridder *R, int32_t transpose, int
         std::shared_ptr<TilePlan> P;
         {
             std::lock_guard<std::mutex> lock(A.plan_mu);
-            P = A.tiles[style][geom];
+            P = A.tiles[style][geom][0];
         }
         CR_REQUIRE(P, CR_INVALID_ARGUMENT, "cr_regrid_copy_tile_plan: the plan is not made (cr_regrid_tile_plan)");
         CR_HIP(hipDeviceSynchronize());

@@ -28,6 +28,9 @@ struct TilePlan {
     DevBuf<int32_t> cols;    // distinct columns of each tile, ascending
     DevBuf<uint16_t> rseg;   // per tile, nrows + 1 entries: first segment of each row
     DevBuf<int32_t> huge;    // rows too long for a tile (done by the long-row kernel)
+    // block-ordered plans: plan row v is matrix row rowmap[v] (-1: none); the
+    // plan's rows are 32-row blocks of the matrix in an order of their own
+    DevBuf<int32_t> rowmap;
     int64_t nhuge = 0;
     int64_t nslots = 0;
 };
@@ -42,15 +45,17 @@ struct Csr {
     int f64_variant = -1;      // FP64 kernel variant; -1: default
     // Kernel plans, made on first use (guarded: handles are shared by callers):
     // the staged-apply tiles, and the rows too long for the row kernels.
-    mutable std::shared_ptr<TilePlan> tiles[2][2];  // per TileStyle and TileGeomId
+    mutable std::shared_ptr<TilePlan> tiles[2][2][2];  // per TileStyle, TileGeomId, block order
     mutable DevBuf<int32_t> long_rows;
     mutable int64_t nlong = -1;  // -1: not planned yet
     mutable std::mutex plan_mu;
 };
 
 // The tile plan of A in the given TileStyle and TileGeomId, made on the device
-// on the first call (synchronises stream s).
-const TilePlan &tile_plan(const Csr &A, int style, int geom, hipStream_t s);
+// on the first call (synchronises stream s).  blocks: plan the rows as 32-row
+// blocks sorted by their smallest column (compact source patches for tiles
+// whose rows are strips of a row-major grid).
+const TilePlan &tile_plan(const Csr &A, int style, int geom, hipStream_t s, bool blocks = false);
 // Forget the plan (after A's values change).
 void drop_tile_plan(const Csr &A);
 

@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(G::kThreads) k_spmm_tiles(
     const typename TileAcc<MODE, T>::load *__restrict__ vals,
     const int32_t *__restrict__ cols, const uint16_t *__restrict__ rseg, const T *__restrict__ x,
     int64_t sc, int64_t sf, T *__restrict__ y, int64_t dc, int64_t df, int64_t nf,
-    const double *__restrict__ scale) {
+    const double *__restrict__ scale, const int32_t *__restrict__ rowmap) {
     using Acc = typename TileAcc<MODE, T>::type;
     using V = typename TileAcc<MODE, T>::val;
     using Off = typename std::conditional<OFF32, uint32_t, int64_t>::type;
@@ -192,15 +192,18 @@ __global__ void __launch_bounds__(G::kThreads) k_spmm_tiles(
 #pragma unroll
     for (int i = 0; i < G::kRowsPerThread; ++i) {
         const int r = tid + i * G::kThreads;
-        rin[i] = r < d.nrows;
+        // the output row: the plan row itself, or its matrix row in a block-ordered plan
+        int64_t orow = d.row0 + r;
+        if (rowmap && r < d.nrows) orow = rowmap[orow];
+        rin[i] = r < d.nrows && orow >= 0;
         sb[i] = 0;
         se[i] = 0;
         rscale[i] = 1.0;
-        yo[i] = rin[i] ? (Off)((d.row0 + r) * dc) : (Off)0;
+        yo[i] = rin[i] ? (Off)(orow * dc) : (Off)0;
         if (rin[i]) {
             sb[i] = rs[r];
             se[i] = rs[r + 1];
-            if (scale) rscale[i] = scale[d.row0 + r];
+            if (scale) rscale[i] = scale[orow];
         }
     }
 
@@ -305,12 +308,12 @@ static void launch_f(const TilePlan &P, int nchunks, int chunk_major, const T *x
         hipLaunchKernelGGL((k_spmm_tiles<G, T, F, MODE, true, UNI>), dim3((unsigned)blocks), dim3(G::kThreads), 0, s,
                            P.tiles.p, P.ntiles, tiles_per_xcd, nchunks, chunk, chunk_major, P.key.p, vals,
                            P.cols.p, P.rseg.p,
-                           x, sc, sf, y, dc, df, nf, scale);
+                           x, sc, sf, y, dc, df, nf, scale, P.rowmap.p);
     else
         hipLaunchKernelGGL((k_spmm_tiles<G, T, F, MODE, false, UNI>), dim3((unsigned)blocks), dim3(G::kThreads), 0, s,
                            P.tiles.p, P.ntiles, tiles_per_xcd, nchunks, chunk, chunk_major, P.key.p, vals,
                            P.cols.p, P.rseg.p,
-                           x, sc, sf, y, dc, df, nf, scale);
+                           x, sc, sf, y, dc, df, nf, scale, P.rowmap.p);
 }
 
 // Workgroups for a tile launch: enough (tile, chunk) pairs to keep every CU
