@@ -917,6 +917,15 @@ void sort_pairs(Temp &T, const K *kin, K *kout, const V *vin, V *vout, int64_t n
     CR_HIP(hipcub::DeviceRadixSort::SortPairs(p, bytes, kin, kout, vin, vout, (int)n, 0, end_bit, s));
 }
 
+// Instantiated here for the planner (plan.hip, planblocks.hip) too: their code
+// objects then carry no device-wide sort or scan of their own, which keeps the
+// first plan's one-time code-object load short.
+template void exclusive_sum<int64_t *, int64_t *>(Temp &, int64_t *, int64_t *, int64_t, hipStream_t);
+template void sort_pairs<int32_t, int32_t>(Temp &, const int32_t *, int32_t *, const int32_t *, int32_t *, int64_t,
+                                           int, hipStream_t);
+template void sort_pairs<uint32_t, int32_t>(Temp &, const uint32_t *, uint32_t *, const int32_t *, int32_t *,
+                                            int64_t, int, hipStream_t);
+
 int bits_for(int64_t n) {
     int b = 1;
     while ((int64_t(1) << b) < n) ++b;

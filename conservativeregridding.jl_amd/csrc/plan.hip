@@ -20,8 +20,9 @@
 //      sort of the tile's column indices), the pieces and their rounds
 //      (kRounds) or the even runs (kEven), every slot, the row -> segment map.
 //
-// The huge rows (more than kCols non-zeros) are listed by a device select.
-#include <hipcub/hipcub.hpp>
+// The huge rows (more than kCols non-zeros) are listed by a flag scan.  The
+// device-wide sort and scan come from build.hip (sort_pairs, exclusive_sum).
+#include <hipcub/backend/rocprim/block/block_radix_sort.hpp>
 
 #include "regridder.hpp"
 
@@ -640,11 +641,17 @@ __global__ void __launch_bounds__(G::kThreads) k_plan_fill(const TileRec *__rest
     }
 }
 
-struct IsHugeRow {
-    const int64_t *rp;
-    int64_t kcols;
-    __host__ __device__ bool operator()(const int32_t &r) const { return rp[r + 1] - rp[r] > kcols; }
-};
+// Rows over kcols non-zeros: flag[r] (0/1), then their ascending list.
+__global__ void k_huge_flags(int64_t nrows, const int64_t *__restrict__ rp, int64_t kcols,
+                             int64_t *__restrict__ flag) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nrows) flag[r] = rp[r + 1] - rp[r] > kcols ? 1 : 0;
+}
+__global__ void k_huge_list(int64_t nrows, const int64_t *__restrict__ rp, int64_t kcols,
+                            const int64_t *__restrict__ pos, int32_t *__restrict__ huge) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nrows && rp[r + 1] - rp[r] > kcols) huge[pos[r]] = (int32_t)r;
+}
 
 }  // namespace
 
@@ -664,36 +671,26 @@ void make_plan(const Csr &A, TilePlan &P, hipStream_t s) {
         DevBuf<int32_t> row(nnz, s), iota(nnz, s), scol(nnz, s), sk(nnz, s);
         hipLaunchKernelGGL(k_plan_rows, dim3(grid_for(nrows, 256)), dim3(256), 0, s, nrows, A.rowptr.p, row.p,
                            iota.p);
-        size_t bytes = 0;
-        CR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, A.col.p, scol.p, iota.p, sk.p, (int)nnz, 0,
-                                                  col_bits, s));
-        void *p = T.get(bytes);
-        CR_HIP(hipcub::DeviceRadixSort::SortPairs(p, bytes, A.col.p, scol.p, iota.p, sk.p, (int)nnz, 0, col_bits,
-                                                  s));
+        sort_pairs<int32_t, int32_t>(T, A.col.p, scol.p, iota.p, sk.p, nnz, col_bits, s);
         hipLaunchKernelGGL(k_plan_prev, dim3(grid_for(nnz, 256)), dim3(256), 0, s, nnz, scol.p, sk.p, row.p,
                            A.rowptr.p, (int64_t)G::kCols, (int64_t)G::kRows, prev.p);
     }
     DevBuf<int2> table(nA > 0 ? nA : 1, s);
     DevBuf<TileRec> recs(nrows > 0 ? nrows : 1, s);
     DevBuf<int64_t> totals(3, s);
-    DevBuf<int32_t> huge(nrows, s), nhuge(1, s);
+    DevBuf<int64_t> hflag(nrows + 1, s), hpos(nrows + 1, s);
     hipLaunchKernelGGL((k_plan_table<G, STYLE>), dim3((unsigned)nA), dim3(kPlanThreads), 0, s, nrows, A.rowptr.p,
                        prev.p, table.p);
     hipLaunchKernelGGL((k_plan_walk<G, STYLE>), dim3(1), dim3(kPlanThreads), 0, s, nrows, A.rowptr.p, prev.p,
                        table.p, recs.p, totals.p);
-    {
-        size_t bytes = 0;
-        IsHugeRow pred{A.rowptr.p, G::kCols};
-        hipcub::CountingInputIterator<int32_t> rows(0);
-        CR_HIP(hipcub::DeviceSelect::If(nullptr, bytes, rows, huge.p, nhuge.p, (int)nrows, pred, s));
-        void *p = T.get(bytes);
-        CR_HIP(hipcub::DeviceSelect::If(p, bytes, rows, huge.p, nhuge.p, (int)nrows, pred, s));
-    }
+    CR_HIP(hipMemsetAsync(hflag.p + nrows, 0, sizeof(int64_t), s));
+    hipLaunchKernelGGL(k_huge_flags, dim3(grid_for(nrows, 256)), dim3(256), 0, s, nrows, A.rowptr.p,
+                       (int64_t)G::kCols, hflag.p);
+    exclusive_sum<int64_t *, int64_t *>(T, hflag.p, hpos.p, nrows + 1, s);
     CR_HIP(hipGetLastError());
-    int64_t h[3];
-    int32_t nh = 0;
+    int64_t h[3], nh = 0;
     CR_HIP(hipMemcpyAsync(h, totals.p, sizeof(h), hipMemcpyDeviceToHost, s));
-    CR_HIP(hipMemcpyAsync(&nh, nhuge.p, sizeof(nh), hipMemcpyDeviceToHost, s));
+    CR_HIP(hipMemcpyAsync(&nh, hpos.p + nrows, sizeof(nh), hipMemcpyDeviceToHost, s));
     CR_HIP(hipStreamSynchronize(s));
     P.ntiles = h[0];
     P.nslots = h[1];
@@ -706,7 +703,8 @@ void make_plan(const Csr &A, TilePlan &P, hipStream_t s) {
     P.rseg.alloc_kept(nrows + P.ntiles, s);
     P.huge.alloc_kept(P.nhuge, s);
     if (P.nhuge)
-        CR_HIP(hipMemcpyAsync(P.huge.p, huge.p, P.nhuge * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_huge_list, dim3(grid_for(nrows, 256)), dim3(256), 0, s, nrows, A.rowptr.p,
+                           (int64_t)G::kCols, hpos.p, P.huge.p);
     DevBuf<int> err(1, s);
     CR_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
     if (P.ntiles)

@@ -8,8 +8,6 @@
 // block is still one 128-byte line of every FP32 output field, and the tile
 // kernel writes plan row v to matrix row rowmap[v].  Its own file: its kernels
 // and sorts load only when a block-ordered plan is first made.
-#include <hipcub/hipcub.hpp>
-
 #include "regridder.hpp"
 
 namespace cr {
@@ -87,20 +85,13 @@ void make_block_plan(const Csr &A, TilePlan &P, hipStream_t s) {
         DevBuf<int32_t> id(nb, s), order(nb, s);
         hipLaunchKernelGGL(k_block_keys, dim3(grid_for(nb, 256)), dim3(256), 0, s, nrows, nb, A.rowptr.p,
                            A.col.p, key.p, id.p);
-        size_t bytes = 0;
-        CR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key.p, skey.p, id.p, order.p, (int)nb, 0, 32,
-                                                  s));
-        void *p = T.get(bytes);
-        CR_HIP(hipcub::DeviceRadixSort::SortPairs(p, bytes, key.p, skey.p, id.p, order.p, (int)nb, 0, 32, s));
+        sort_pairs<uint32_t, int32_t>(T, key.p, skey.p, id.p, order.p, nb, 32, s);
         DevBuf<int64_t> n(nv + 1, s);
         CR_HIP(hipMemsetAsync(n.p + nv, 0, sizeof(int64_t), s));
         hipLaunchKernelGGL(k_block_rows, dim3(grid_for(nv, 256)), dim3(256), 0, s, nrows, nv, order.p, A.rowptr.p,
                            rowmap.p, n.p);
         B.rowptr.alloc_async(nv + 1, s);
-        bytes = 0;
-        CR_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, n.p, B.rowptr.p, (int)(nv + 1), s));
-        p = T.get(bytes);
-        CR_HIP(hipcub::DeviceScan::ExclusiveSum(p, bytes, n.p, B.rowptr.p, (int)(nv + 1), s));
+        exclusive_sum<int64_t *, int64_t *>(T, n.p, B.rowptr.p, nv + 1, s);
     }
     B.col.alloc_async(nnz > 0 ? nnz : 1, s);
     B.val.alloc_async(nnz > 0 ? nnz : 1, s);

@@ -92,6 +92,12 @@ struct Temp {
         return buf.p;
     }
 };
+// Device-wide hipcub exclusive sum and stable radix sort of (key, value) pairs
+// on the bits [0, end_bit), with T's temporary storage (build.hip).
+template <class In, class Out>
+void exclusive_sum(Temp &T, In in, Out out, int64_t n, hipStream_t s);
+template <class K, class V>
+void sort_pairs(Temp &T, const K *kin, K *kout, const V *vin, V *vout, int64_t n, int end_bit, hipStream_t s);
 // CSR(A^T) of the device CSR(A) F whose entry k lies in row coo_row[k].
 void csr_transpose(const Csr &F, const int32_t *coo_row, Csr &Bk, Temp &T, hipStream_t s);
 }  // namespace cr
