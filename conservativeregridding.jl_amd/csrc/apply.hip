@@ -300,6 +300,24 @@ constexpr F32Variant kF32Variants[] = {
     {true, 4, 2, 4, 0, 1},       {true, 4, 2, 0, 0, 1, 1},    {true, 4, 2, 4, 0, 1, 1}};
 constexpr int kNumVariants = sizeof(kF32Variants) / sizeof(kF32Variants[0]);
 constexpr int kDefaultF32Variant = 21;  // 1024-row tiles, 4 fields per stage, FP64 sums of FP32 A
+constexpr int kBlockF32Variant = 23;    // the same on the block-ordered plan
+
+// The default FP32 variant of a matrix, chosen once: the block-ordered plan
+// when the plain plan's tiles fetch more than 3 source lines (32 columns each)
+// per 32 distinct columns and the block-ordered one fetches fewer (rows that
+// are strips of a row-major grid over sources numbered in another order).
+static int default_f32_variant(const Csr &A, hipStream_t s) {
+    int v = A.default_variant.load();
+    if (v >= 0) return v;
+    v = kDefaultF32Variant;
+    const TilePlan &P = tile_plan(A, kRounds, kGeomL, s, false);
+    if (P.tile_cols > 0 && P.tile_lines * 32 > 3 * P.tile_cols) {
+        const TilePlan &Q = tile_plan(A, kRounds, kGeomL, s, true);
+        if (Q.tile_cols > 0 && Q.tile_lines * P.tile_cols < P.tile_lines * Q.tile_cols) v = kBlockF32Variant;
+    }
+    A.default_variant.store(v);
+    return v;
+}
 
 static void launch_f32_variant(int v, const Csr &A, const float *x, int64_t sc, int64_t sf, float *y,
                                int64_t dc, int64_t df, int64_t nf, const double *inv, hipStream_t s) {
@@ -347,7 +365,7 @@ void apply(const cr_regridder *R, bool transpose, int dtype, int64_t nf, const v
                            normalize ? A.inv_area.p : nullptr);
     } else {
         int v = force_variant >= 0 ? force_variant : A.tuned_variant;
-        if (v < 0 || v >= kNumVariants) v = kDefaultF32Variant;
+        if (v < 0 || v >= kNumVariants) v = default_f32_variant(A, s);
         launch_f32_variant(v, A, (const float *)src, sc, sf, (float *)dst, dc, df, nf,
                            normalize ? A.inv_area.p : nullptr, s);
     }

@@ -330,7 +330,8 @@ __global__ void __launch_bounds__(G::kThreads) k_plan_fill(const TileRec *__rest
                                                            const double *__restrict__ val, TileDesc *__restrict__ desc,
                                                            uint32_t *__restrict__ keys, float *__restrict__ v32,
                                                            double *__restrict__ v64, int32_t *__restrict__ cols,
-                                                           uint16_t *__restrict__ rseg, int *__restrict__ err) {
+                                                           uint16_t *__restrict__ rseg, int *__restrict__ err,
+                                                           unsigned long long *__restrict__ stats) {
     constexpr int NT = G::kThreads, RPT = G::kRows / NT, IPT = G::kEnt, ZMAX = NT * IPT;
     using Sort = hipcub::BlockRadixSort<uint32_t, NT, IPT, uint16_t>;
     // the sort's storage is free once every thread is past the barrier after it
@@ -414,13 +415,16 @@ __global__ void __launch_bounds__(G::kThreads) k_plan_fill(const TileRec *__rest
     bool first[IPT];
     {
         uint32_t pk = tid > 0 ? lastkey[tid - 1] : 0xffffffffu;
+        uint32_t nlines = 0;  // 32-column lines (128 bytes of an FP32 field) the tile's columns start
 #pragma unroll
         for (int i = 0; i < IPT; ++i) {
             const int pos = tid * IPT + i;
             first[i] = pos < Z && (pos == 0 || key[i] != pk);
             fl[0] += first[i] ? 1u : 0u;
+            nlines += (pos < Z && (pos == 0 || (key[i] >> 5) != (pk >> 5))) ? 1u : 0u;
             pk = key[i];
         }
+        if (nlines) atomicAdd(stats + 1, (unsigned long long)nlines);
     }
     block_scan<1, NT>(fl, ftot, scan_lds);
     {
@@ -439,6 +443,7 @@ __global__ void __launch_bounds__(G::kThreads) k_plan_fill(const TileRec *__rest
     }
     __syncthreads();
     const int ncols = (int)ftot[0];
+    if (tid == 0) atomicAdd(stats, (unsigned long long)ncols);
     if (ncols > R.cap) {
         if (tid == 0) err[0] = 1;
         return;
@@ -705,17 +710,20 @@ void make_plan(const Csr &A, TilePlan &P, hipStream_t s) {
     if (P.nhuge)
         hipLaunchKernelGGL(k_huge_list, dim3(grid_for(nrows, 256)), dim3(256), 0, s, nrows, A.rowptr.p,
                            (int64_t)G::kCols, hpos.p, P.huge.p);
-    DevBuf<int> err(1, s);
-    CR_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
+    // err, then the tiles' distinct columns and the 32-column lines they start
+    DevBuf<unsigned long long> fstat(3, s);
+    CR_HIP(hipMemsetAsync(fstat.p, 0, fstat.bytes(), s));
     if (P.ntiles)
         hipLaunchKernelGGL((k_plan_fill<G, STYLE>), dim3((unsigned)P.ntiles), dim3(G::kThreads), 0, s, recs.p,
                            col_bits, A.rowptr.p, A.col.p, A.val.p, P.tiles.p, P.key.p, P.val32.p, P.val64.p,
-                           P.cols.p, P.rseg.p, err.p);
+                           P.cols.p, P.rseg.p, (int *)fstat.p, fstat.p + 1);
     CR_HIP(hipGetLastError());
-    int bad = 0;
-    CR_HIP(hipMemcpyAsync(&bad, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    unsigned long long f[3] = {0, 0, 0};
+    CR_HIP(hipMemcpyAsync(f, fstat.p, sizeof(f), hipMemcpyDeviceToHost, s));
     CR_HIP(hipStreamSynchronize(s));
-    CR_REQUIRE(!bad, CR_INTERNAL, "tile plan: a tile broke its budgets");
+    CR_REQUIRE(!f[0], CR_INTERNAL, "tile plan: a tile broke its budgets");
+    P.tile_cols = (int64_t)f[1];
+    P.tile_lines = (int64_t)f[2];
 }
 
 template void make_plan<GeomS, kRounds>(const Csr &, TilePlan &, hipStream_t);

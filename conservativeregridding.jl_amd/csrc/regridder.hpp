@@ -10,6 +10,7 @@
 //   counterpart: cr_regrid takes strides.
 #pragma once
 
+#include <atomic>
 #include <memory>
 #include <mutex>
 
@@ -33,6 +34,10 @@ struct TilePlan {
     DevBuf<int32_t> rowmap;
     int64_t nhuge = 0;
     int64_t nslots = 0;
+    // the tiles' distinct columns and the 32-column lines (128 bytes of an FP32
+    // field) they touch, summed over tiles: lines * 32 / cols is the plan's
+    // source-line fetch per source value used
+    int64_t tile_cols = 0, tile_lines = 0;
 };
 
 struct Csr {
@@ -42,6 +47,7 @@ struct Csr {
     DevBuf<double> val;
     DevBuf<double> inv_area;   // 1 / area of the row cells (FP32 batched path)
     int tuned_variant = -1;    // FP32 kernel variant (cr_regrid_tune / set_variant); -1: default
+    mutable std::atomic<int> default_variant{-1};  // the default FP32 variant chosen for this matrix; -1: not yet
     int f64_variant = -1;      // FP64 kernel variant; -1: default
     // Kernel plans, made on first use (guarded: handles are shared by callers):
     // the staged-apply tiles, and the rows too long for the row kernels.

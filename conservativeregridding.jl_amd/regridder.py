@@ -465,8 +465,10 @@ def regrid_set_variant(R: Regridder, variant: int, dtype="float32") -> None:
     fields per thread, 5-7 staged tile kernel with 2/4/8 fields per stage and
     Kahan sums, 8-10 the same with FP64 sums, 11-14 variant 9 with other field
     chunkings, 15-17 FP64 sums on 1024-row tiles, 18-19 Kahan sums on 1024-row
-    tiles, 20-21 FP64 sums of A rounded to FP32 on 1024-row tiles (default 21),
-    22-23 the same on 32-row blocks sorted by their smallest column;
+    tiles, 20-21 FP64 sums of A rounded to FP32 on 1024-row tiles, 22-23 the
+    same on 32-row blocks sorted by their smallest column (default: 21, or 23
+    when its plan fetches fewer source lines than 21's and 21's fetch over 3
+    lines per 32 distinct source cells);
     FP64: 0 row kernel, 1 tile kernel (default)."""
     L.check(L.lib.cr_regrid_set_variant(R._core.h, 1 if R._t else 0, _dtype_code(dtype), int(variant)),
             "regrid_set_variant")

@@ -264,9 +264,12 @@ cr_status cr_regrid_tune(cr_regridder *R, int32_t transpose, int64_t nfields, co
  *           4 field chunks); 18-19 FP32 Kahan sums on 1024-row tiles with 4
  *           fields per stage (19: 4 field chunks); 20-21 FP64 sums of A
  *           rounded to FP32 on 1024-row tiles, 4 fields per stage (21: 4 field
- *           chunks; the default); 22-23 variants 20-21 on a plan whose rows
- *           are 32-row blocks sorted by their smallest column (for rows that
- *           are strips of a row-major grid over a source in another order).  FP32 fields with unit field strides and >= 64
+ *           chunks); 22-23 variants 20-21 on a plan whose rows are 32-row
+ *           blocks sorted by their smallest column (for rows that are strips
+ *           of a row-major grid over a source in another order).  The default
+ *           is 21, or 23 when 21's tiles fetch over 3 source lines per 32
+ *           distinct source cells and 23's fetch fewer (decided once per
+ *           direction at its first default apply).  FP32 fields with unit field strides and >= 64
  *           fields always use the fields-fastest kernel.
  *   CR_F64: 0 row kernel, 1 staged tile kernel (default).
  */
