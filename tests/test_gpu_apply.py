@@ -272,6 +272,33 @@ def test_every_kernel_variant_matches_reference(cr, k):
         assert np.all(np.isfinite(ref))
 
 
+def test_default_variant_on_row_strips_over_nested_sources(cr):
+    """Lon-lat destination rows (strips of a row-major grid) over HEALPix nested
+    sources: the default FP32 apply (which may pick the block-ordered plan for
+    such a matrix) and the block-ordered variants give the FP64 result to the
+    FP32 tolerance, both directions, every destination row written."""
+    _case("row_strips", src="healpix64 nested", dst="1deg lon-lat")
+    src = cr.Grid.healpix(64, "nested", step=1)
+    dst = cr.Grid.from_points(cr.lonlat_points(360, 180))
+    R = cr.Regridder(dst, src, radius=1.0)
+    rng = np.random.default_rng(3)
+    k = 12
+    for RR in (R, cr.transpose(R)):
+        n_out, n_in = RR.shape
+        x64 = torch.tensor(rng.random((k, n_in)), dtype=torch.float64, device=DEV)
+        ref = torch.zeros(k, n_out, dtype=torch.float64, device=DEV)
+        cr.regrid_(ref.T, RR, x64.T)
+        x = x64.float()
+        for v in (-1, 22, 23):
+            cr.regrid_set_variant(RR, v)
+            y = torch.full((k, n_out), float("nan"), dtype=torch.float32, device=DEV)
+            cr.regrid_(y.T, RR, x.T)
+            assert torch.isfinite(y).all(), v
+            err = ((y.double() - ref).abs() / ref.abs().clamp_min(1e-30)).max().item()
+            assert err <= 1e-6, (v, err)
+        cr.regrid_set_variant(RR, -1)
+
+
 def test_tile_kernel_on_c2_fields(cr, c2):
     _case("tiles_c2")
     R, A = c2
