@@ -18,7 +18,9 @@
 //   4. one readback of the totals, the plan's buffers allocated;
 //   5. k_plan_fill: one workgroup per tile: the distinct columns (block radix
 //      sort of the tile's column indices), the pieces and their rounds
-//      (kRounds) or the even runs (kEven), every slot, the row -> segment map.
+//      (kRounds) or the even runs (kEven), every slot, the row -> segment map;
+//      it also sums the tiles' distinct columns and the 32-column lines they
+//      start (the apply's choice between plain and block-ordered plans).
 //
 // The huge rows (more than kCols non-zeros) are listed by a flag scan.  The
 // device-wide sort and scan come from build.hip (sort_pairs, exclusive_sum).
