@@ -140,6 +140,7 @@ struct cr_grid {
     int64_t nverts = 0;          // caller-visible vertices (points of a point grid)
     cr::DevBuf<double> cells;    // ncells * nv * 3, xyz interleaved per vertex
     cr::DevBuf<int32_t> vidx;    // ncells * nv: caller vertex of each cell slot
+    int32_t private_vertices = 0;  // 1: no vertex index is shared by two cells (tables of cells)
     cr::DevBuf<int32_t> fold;    // 2 * nfold: (primary, partner) cells of a folded row
     int64_t nfold = 0;
 };

@@ -177,12 +177,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
     }
 }
 
-// d(area of each destination cell)/dV times dL/da_u.
+// d(area of each destination cell)/dV times dL/da_u.  private_v: no vertex
+// index is shared by two cells, so each thread owns its cell's entries of g
+// and adds without atomics (the kernels before it on the stream have finished).
 __global__ void k_area_grad(const double *__restrict__ cells, int nv, const int32_t *__restrict__ vi,
                             int64_t ncells, double kappa, const double *__restrict__ a_to, int64_t nf,
                             const double *__restrict__ y, int64_t yc, int64_t yf,
                             const double *__restrict__ gy, int64_t gc, int64_t gf,
-                            double *__restrict__ g) {
+                            double *__restrict__ g, int private_v) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ncells || a_to[t] == 0.0) return;  // zero-area (ghost) cells have no area term
     double s = 0.0;
@@ -191,9 +193,16 @@ __global__ void k_area_grad(const double *__restrict__ cells, int nv, const int3
     if (c == 0.0) return;
     const int32_t *v = vi + (int64_t)nv * t;
     ring_area_backward(cells + 3 * (int64_t)nv * t, nv, c, [&](int slot, const d3 &d) {
-        atomicAdd(g + 3 * (int64_t)v[slot], d.x);
-        atomicAdd(g + 3 * (int64_t)v[slot] + 1, d.y);
-        atomicAdd(g + 3 * (int64_t)v[slot] + 2, d.z);
+        double *gv = g + 3 * (int64_t)v[slot];
+        if (private_v) {
+            gv[0] += d.x;
+            gv[1] += d.y;
+            gv[2] += d.z;
+        } else {
+            atomicAdd(gv, d.x);
+            atomicAdd(gv + 1, d.y);
+            atomicAdd(gv + 2, d.z);
+        }
     });
 }
 
@@ -263,7 +272,8 @@ cr_status cr_regrid_backward(const cr_regridder *R, const cr_grid *dst, const cr
         const cr_grid *gto = transpose ? src : dst;
         if (normalize && g_to)
             hipLaunchKernelGGL(k_area_grad, dim3(grid_for(n_to, B)), dim3(B), 0, s, gto->cells.p, gto->nv,
-                               gto->vidx.p, n_to, kappa, a_to, nfields, y, yc, yf, gy, gc, gf, g_to);
+                               gto->vidx.p, n_to, kappa, a_to, nfields, y, yc, yf, gy, gc, gf, g_to,
+                               gto->private_vertices);
         CR_HIP(hipGetLastError());
         CR_HIP(hipStreamSynchronize(s));
     });

@@ -304,6 +304,7 @@ cr_status cr_grid_from_cells(cr_grid **grid, const double *cells, int64_t ncells
             g->vidx.alloc((size_t)ncells * nv);
             hipLaunchKernelGGL(k_iota_i32, dim3(grid_for(g->nverts, 256)), dim3(256), 0, s, g->vidx.p,
                                g->nverts);
+            g->private_vertices = 1;  // orientation only permutes a cell's own slots
             orient_grid(g, s);
         } catch (...) {
             delete g;
@@ -467,6 +468,7 @@ cr_status cr_grid_healpix(cr_grid **grid, int64_t nside, int32_t order, int32_t 
             g->nverts = nthr;
             g->vidx.alloc((size_t)nthr);
             hipLaunchKernelGGL(k_iota_i32, dim3(grid_for(nthr, 256)), dim3(256), 0, s, g->vidx.p, nthr);
+            g->private_vertices = 1;
             CR_HIP(hipGetLastError());
             CR_HIP(hipStreamSynchronize(s));
         } catch (...) {

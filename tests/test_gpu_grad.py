@@ -145,3 +145,32 @@ def test_autograd_function(cr, setup):
     g = Pt.grad.numpy()
     radial = np.abs(np.sum(g * P, axis=-1))
     assert radial.max() <= 1e-10 * np.abs(g).max()
+
+
+def test_cell_table_gradients_sum_to_point_gradients(cr, setup):
+    """A destination given as a table of cells owns one vertex per cell slot
+    (no vertex shared between cells: its area term adds without atomics); the
+    slot gradients summed over the slots that are one point of the point grid
+    give that grid's point gradient."""
+    _case("grad_cell_table")
+    P, D, s, t = setup
+    src = cr.Grid.from_points(P)
+    ni, nj = D.shape[0] - 1, D.shape[1] - 1
+    grads = []
+    for dst in (cr.Grid.from_points(D), cr.Grid.from_cells(cr.points_to_cells(D), convex=True)):
+        R = cr.Regridder(dst, src, radius=1.0)
+        st = torch.tensor(s, device=DEV)
+        d = cr.regrid(R, st)
+        gy = 2.0 * (d - torch.tensor(t, device=DEV))
+        _, g_dst, _ = cr.regrid_backward(R, dst, src, st, d, gy, field_grad=False)
+        grads.append(g_dst.cpu().numpy().reshape(-1, 3))
+    g_pts, g_slots = grads
+    # slot k of cell (i, j) is point (i, j), (i+1, j), (i+1, j+1), (i, j+1)
+    i, j = np.meshgrid(np.arange(ni), np.arange(nj), indexing="xy")
+    i, j = i.reshape(-1), j.reshape(-1)  # cell index i + j * ni
+    corner = np.stack([i + j * (ni + 1), i + 1 + j * (ni + 1), i + 1 + (j + 1) * (ni + 1),
+                       i + (j + 1) * (ni + 1)], axis=1).reshape(-1)
+    summed = np.zeros_like(g_pts)
+    np.add.at(summed, corner, g_slots)
+    scale = np.abs(g_pts).max()
+    np.testing.assert_allclose(summed, g_pts, rtol=0, atol=1e-10 * scale)
