@@ -17,25 +17,29 @@
 
 namespace cr {
 
-// Per non-zero of CSR(A) (rows = dst, cols = src): the coefficient dL/dA_u,ij
-// (unit-sphere areas) and the row index.
-__global__ void k_pair_coef(const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
-                            int64_t nrows, int transpose, int normalize, double kappa,
+// Row index of every non-zero of CSR(A) (rows = dst).
+__global__ void k_pair_rows(const int64_t *__restrict__ rowptr, int64_t nrows, int32_t *__restrict__ rowid) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows) return;
+    for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) rowid[k] = (int32_t)i;
+}
+
+// Per non-zero of CSR(A) (rows = dst, cols = src), one thread each: the
+// coefficient dL/dA_u,ij (unit-sphere areas).
+__global__ void k_pair_coef(const int32_t *__restrict__ rowid, const int32_t *__restrict__ col, int64_t nnz,
+                            int transpose, int normalize, double kappa,
                             const double *__restrict__ a_to, int64_t nf,
                             const double *__restrict__ x, int64_t xc, int64_t xf,
                             const double *__restrict__ gy, int64_t gc, int64_t gf,
-                            double *__restrict__ coef, int32_t *__restrict__ rowid) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nrows) return;
-    for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
-        int64_t j = col[k];
-        // forward: y lives on dst rows i, x on src cols j; transpose: the reverse
-        int64_t to = transpose ? j : i, from = transpose ? i : j;
-        double s = 0.0;
-        for (int64_t f = 0; f < nf; ++f) s += gy[to * gc + f * gf] * x[from * xc + f * xf];
-        coef[k] = normalize ? kappa * s / a_to[to] : kappa * s;
-        rowid[k] = (int32_t)i;
-    }
+                            double *__restrict__ coef) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    const int64_t i = rowid[k], j = col[k];
+    // forward: y lives on dst rows i, x on src cols j; transpose: the reverse
+    const int64_t to = transpose ? j : i, from = transpose ? i : j;
+    double s = 0.0;
+    for (int64_t f = 0; f < nf; ++f) s += gy[to * gc + f * gf] * x[from * xc + f * xf];
+    coef[k] = normalize ? kappa * s / a_to[to] : kappa * s;
 }
 
 // Scatter target of the clip backward: caller-vertex gradients of both grids.
@@ -260,9 +264,11 @@ cr_status cr_regrid_backward(const cr_regridder *R, const cr_grid *dst, const cr
         const double kappa = R->area_scale;
         DevBuf<double> coef(F.nnz, s);
         DevBuf<int32_t> rowid(F.nnz, s);
-        hipLaunchKernelGGL(k_pair_coef, dim3(grid_for(F.nrows, B)), dim3(B), 0, s, F.rowptr.p, F.col.p, F.nrows,
-                           (int)(transpose != 0), (int)(normalize != 0), kappa, a_to, nfields, x, xc, xf, gy,
-                           gc, gf, coef.p, rowid.p);
+        hipLaunchKernelGGL(k_pair_rows, dim3(grid_for(F.nrows, B)), dim3(B), 0, s, F.rowptr.p, F.nrows, rowid.p);
+        if (F.nnz)
+            hipLaunchKernelGGL(k_pair_coef, dim3(grid_for(F.nnz, B)), dim3(B), 0, s, rowid.p, F.col.p, F.nnz,
+                               (int)(transpose != 0), (int)(normalize != 0), kappa, a_to, nfields, x, xc, xf, gy,
+                               gc, gf, coef.p);
         if (F.nnz)
             hipLaunchKernelGGL(k_clip_grad, dim3(grid_for(F.nnz, 64)), dim3(64),
                                (size_t)kGradRows * dst->nv * 3 * sizeof(double), s, rowid.p, F.col.p, coef.p,
