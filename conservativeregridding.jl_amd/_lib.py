@@ -110,7 +110,8 @@ SIGNATURES = {
                               ctypes.POINTER(EsmfAttrs)]),
     "cr_regrid_set_variant": (c_i32, [c_vp, c_i32, c_i32, c_i32]),
     "cr_regrid_tile_plan": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp]),
-    "cr_regrid_copy_tile_plan": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "cr_regrid_copy_tile_plan": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                         c_vp]),
 }
 
 

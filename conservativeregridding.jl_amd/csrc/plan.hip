@@ -770,34 +770,35 @@ cr_status cr_regrid_tile_plan(const cr_regridder *R, int32_t transpose, int32_t 
                               int64_t *sizes, void *stream) {
     return guard([&] {
         CR_REQUIRE(R && sizes, CR_INVALID_ARGUMENT, "cr_regrid_tile_plan: null argument");
-        CR_REQUIRE((style == kRounds || style == kEven) && (geom == kGeomS || geom == kGeomL), CR_INVALID_ARGUMENT,
-                   "cr_regrid_tile_plan: style and geom are 0 or 1");
+        CR_REQUIRE(style >= 0 && style <= 3 && (geom == kGeomS || geom == kGeomL), CR_INVALID_ARGUMENT,
+                   "cr_regrid_tile_plan: style is 0..3 and geom 0 or 1");
         const Csr &A = transpose ? R->bwd : R->fwd;
         hipStream_t s = as_stream(stream);
-        for (int i = 0; i < 5; ++i) sizes[i] = 0;
+        for (int i = 0; i < 6; ++i) sizes[i] = 0;
         if (A.nrows == 0) return;
-        const TilePlan &P = tile_plan(A, style, geom, s);
+        const TilePlan &P = tile_plan(A, style & 1, geom, s, (style & 2) != 0);
         CR_HIP(hipStreamSynchronize(s));
         sizes[0] = P.ntiles;
         sizes[1] = P.nslots;
         sizes[2] = (int64_t)P.cols.n;
         sizes[3] = (int64_t)P.rseg.n;
         sizes[4] = P.nhuge;
+        sizes[5] = (int64_t)P.rowmap.n;
     });
 }
 
 cr_status cr_regrid_copy_tile_plan(const cr_regridder *R, int32_t transpose, int32_t style, int32_t geom,
                                    void *desc, uint32_t *key, float *v32, double *v64, int32_t *cols,
-                                   uint16_t *rseg, int32_t *huge) {
+                                   uint16_t *rseg, int32_t *huge, int32_t *rowmap) {
     return guard([&] {
         CR_REQUIRE(R, CR_INVALID_ARGUMENT, "cr_regrid_copy_tile_plan: null regridder");
-        CR_REQUIRE((style == kRounds || style == kEven) && (geom == kGeomS || geom == kGeomL), CR_INVALID_ARGUMENT,
-                   "cr_regrid_copy_tile_plan: style and geom are 0 or 1");
+        CR_REQUIRE(style >= 0 && style <= 3 && (geom == kGeomS || geom == kGeomL), CR_INVALID_ARGUMENT,
+                   "cr_regrid_copy_tile_plan: style is 0..3 and geom 0 or 1");
         const Csr &A = transpose ? R->bwd : R->fwd;
         std::shared_ptr<TilePlan> P;
         {
             std::lock_guard<std::mutex> lock(A.plan_mu);
-            P = A.tiles[style][geom][0];
+            P = A.tiles[style & 1][geom][(style & 2) ? 1 : 0];
         }
         CR_REQUIRE(P, CR_INVALID_ARGUMENT, "cr_regrid_copy_tile_plan: the plan is not made (cr_regrid_tile_plan)");
         CR_HIP(hipDeviceSynchronize());
@@ -814,6 +815,7 @@ cr_status cr_regrid_copy_tile_plan(const cr_regridder *R, int32_t transpose, int
         down(cols, P->cols.p, P->cols.bytes());
         down(rseg, P->rseg.p, P->rseg.bytes());
         down(huge, P->huge.p, P->huge.bytes());
+        down(rowmap, P->rowmap.p, P->rowmap.bytes());
     });
 }
 

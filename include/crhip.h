@@ -278,24 +278,28 @@ cr_status cr_regrid_set_variant(cr_regridder *R, int32_t transpose, int32_t dtyp
 
 /*
  * Tile plans of the staged apply (no reference counterpart).  The tile kernel
- * runs on a plan of one direction of R, in layout `style` (0: rounds, used by
- * FP32 fields; 1: even runs, used by FP64 fields) and geometry `geom` (0: 256
- * threads, 512 rows, 1024 columns per tile; 1: 512, 1024, 2048).  The first
- * cr_regrid that needs a plan makes it on the device; cr_regrid_tile_plan makes
- * it ahead of time (for instance before capturing cr_regrid in a HIP graph) and
- * gives its sizes: sizes[0] tiles, [1] slots, [2] column-list entries, [3]
- * row-map entries, [4] rows left to the long-row kernel.  Synchronises the
- * stream.  cr_regrid_copy_tile_plan copies a made plan to host memory:
- * descriptors (72 bytes per tile: row0, ent_off, col_off as int64; nrows,
- * ncols, ent, nseg as int32; 8 uint32 flush masks), slot keys (local column |
- * segment << 16), slot values as FP32 and FP64, column lists, row maps and
- * long rows, each buffer holding the corresponding sizes[] entries.
+ * runs on a plan of one direction of R, in layout style & 1 (0: rounds, used by
+ * FP32 fields; 1: even runs, used by FP64 fields), over the matrix rows in
+ * order (style & 2 == 0) or over 32-row blocks sorted by their smallest column
+ * (style & 2; FP32 variants 22-23), and geometry `geom` (0: 256 threads, 512
+ * rows, 1024 columns per tile; 1: 512, 1024, 2048).  The first cr_regrid that
+ * needs a plan makes it on the device; cr_regrid_tile_plan makes it ahead of
+ * time (for instance before capturing cr_regrid in a HIP graph) and gives its
+ * sizes: sizes[0] tiles, [1] slots, [2] column-list entries, [3] segment-map
+ * entries, [4] rows left to the long-row kernel, [5] row-map entries (block
+ * order only).  Synchronises the stream.  cr_regrid_copy_tile_plan copies a
+ * made plan to host memory: descriptors (72 bytes per tile: row0, ent_off,
+ * col_off as int64; nrows, ncols, ent, nseg as int32; 8 uint32 flush masks),
+ * slot keys (local column | segment << 16), slot values as FP32 and FP64,
+ * column lists, segment maps, long rows (matrix rows) and the row map (plan
+ * row -> matrix row, -1 for none), each buffer holding the corresponding
+ * sizes[] entries.
  */
 cr_status cr_regrid_tile_plan(const cr_regridder *R, int32_t transpose, int32_t style, int32_t geom,
                               int64_t *sizes, void *stream);
 cr_status cr_regrid_copy_tile_plan(const cr_regridder *R, int32_t transpose, int32_t style, int32_t geom,
                                    void *desc, uint32_t *key, float *v32, double *v64, int32_t *cols,
-                                   uint16_t *rseg, int32_t *huge);
+                                   uint16_t *rseg, int32_t *huge, int32_t *rowmap);
 
 /* ------------------------------------------------- ESMF weights files -- */
 /*

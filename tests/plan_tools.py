@@ -126,16 +126,42 @@ def emulate(P, style, rp, col, val, x, threads):
     return y
 
 
-def device_plan(L, handle, transpose, style, geom, empty_plan):
-    """The device plan of one direction of a regridder handle, as host arrays."""
-    sizes = np.zeros(5, np.int64)
-    L.check(L.lib.cr_regrid_tile_plan(handle, int(transpose), int(style), int(geom), sizes.ctypes.data, None),
-            "tile_plan")
-    P = empty_plan(sizes)
-    L.check(L.lib.cr_regrid_copy_tile_plan(handle, int(transpose), int(style), int(geom),
+def device_plan(L, handle, transpose, style, geom, empty_plan, blocks=False):
+    """The device plan of one direction of a regridder handle, as host arrays
+    (blocks: the plan over 32-row blocks sorted by their smallest column, with
+    its row map)."""
+    sizes = np.zeros(6, np.int64)
+    st = int(style) | (2 if blocks else 0)
+    L.check(L.lib.cr_regrid_tile_plan(handle, int(transpose), st, int(geom), sizes.ctypes.data, None), "tile_plan")
+    P = empty_plan(sizes[:5])
+    P["rowmap"] = np.zeros(int(sizes[5]), np.int32)
+    L.check(L.lib.cr_regrid_copy_tile_plan(handle, int(transpose), st, int(geom),
                                            *(P[k].ctypes.data for k in ("desc", "key", "v32", "v64", "cols",
-                                                                        "rseg", "huge"))), "copy_tile_plan")
+                                                                        "rseg", "huge", "rowmap"))),
+            "copy_tile_plan")
     return P
+
+
+def block_order(rp, col):
+    """The 32-row block order of a block-ordered plan, restated: blocks sorted
+    by their smallest column (stable; a block with no non-zeros sorts last),
+    plan row v = matrix row 32 * order[v // 32] + v % 32, or -1 past the last
+    row.  Returns (rowmap, the block-ordered CSR (rp, col, val index))."""
+    nrows = rp.shape[0] - 1
+    nb = (nrows + 31) // 32
+    key = np.full(nb, 0xFFFFFFFF, np.uint64)
+    for b in range(nb):
+        k0, k1 = rp[32 * b], rp[min(nrows, 32 * b + 32)]
+        if k1 > k0:
+            key[b] = col[k0:k1].min()
+    order = np.argsort(key, kind="stable")
+    v = np.arange(nb * 32)
+    rowmap = order[v // 32] * 32 + v % 32
+    rowmap = np.where(rowmap < nrows, rowmap, -1).astype(np.int32)
+    n = np.where(rowmap >= 0, np.diff(rp)[np.maximum(rowmap, 0)], 0)
+    rp2 = np.concatenate([[0], np.cumsum(n)]).astype(np.int64)
+    idx = np.concatenate([np.arange(rp[r], rp[r + 1]) for r in rowmap if r >= 0] or [np.zeros(0, np.int64)])
+    return rowmap, rp2, idx.astype(np.int64)
 
 
 def raw_regridder(L, nrows, ncols, rp, col, val):

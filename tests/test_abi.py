@@ -63,8 +63,8 @@ def test_last_error_is_thread_local_text(cr):
 def test_tile_plan_entry_points_reject_bad_arguments(cr):
     """Host-side argument checks of the tile-plan entry points (no device call)."""
     L = cr._lib
-    sizes = (ctypes.c_int64 * 5)()
+    sizes = (ctypes.c_int64 * 6)()
     assert L.lib.cr_regrid_tile_plan(None, 0, 0, 0, sizes, None) == L.CR_INVALID_ARGUMENT
-    assert L.lib.cr_regrid_copy_tile_plan(None, 0, 0, 0, None, None, None, None, None, None, None) \
+    assert L.lib.cr_regrid_copy_tile_plan(None, 0, 0, 0, None, None, None, None, None, None, None, None) \
         == L.CR_INVALID_ARGUMENT
     assert "null" in L.last_error()

@@ -12,7 +12,8 @@ device plan, run through the kernel's contract on the host, must give A x."""
 import numpy as np
 import pytest
 
-from plan_tools import EVEN, ROUNDS, assert_same_plan, device_plan, emulate, raw_regridder, synthetic_csr
+from plan_tools import (EVEN, ROUNDS, assert_same_plan, block_order, device_plan, emulate, raw_regridder,
+                        synthetic_csr)
 
 pytestmark = pytest.mark.gpu
 
@@ -53,3 +54,23 @@ def test_device_plan_c48(cr, orc):
                                    ref[k][d["col_off"]:d["col_off"] + d["ncols"]]) for d in ref["desc"]), k
             y = emulate(dev, style, rp, col, val, x, orc.TILE_GEOMS[geom][0])
             np.testing.assert_allclose(y, want, rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("kind", ["banded", "ragged", "wide"])
+def test_device_block_plan_matches_oracle(cr, orc, kind):
+    """Block-ordered plans: the device's row map equals the restated block
+    order, and its plan equals the oracle's plan of the block-ordered matrix
+    bit for bit (long rows as matrix rows)."""
+    nrows, ncols, rp, col, val = synthetic_csr(kind, seed=6)
+    h = raw_regridder(cr._lib, nrows, ncols, rp, col, val)
+    try:
+        rowmap, rp2, idx = block_order(rp, col)
+        for style, geom in COMBOS:
+            print(f"[case] block plan {kind} style={style} geom={geom}", flush=True)
+            dev = device_plan(cr._lib, h, 0, style, geom, orc.empty_tile_plan, blocks=True)
+            np.testing.assert_array_equal(dev["rowmap"], rowmap)
+            ref = orc.tile_plan(style, geom, rp2, col[idx], val[idx], ncols)
+            ref["huge"] = rowmap[ref["huge"]]
+            assert_same_plan({k: v for k, v in dev.items() if k != "rowmap"}, ref)
+    finally:
+        cr._lib.lib.cr_regridder_destroy(h)
