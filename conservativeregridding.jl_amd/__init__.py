@@ -5,7 +5,8 @@ Loaded from its folder by path (the folder name is not a Python identifier):
 ``__graft_entry__.load_package()`` returns this module.
 """
 from ._lib import CRError, LIB_PATH, lib as _lib_handle
-from .errors import ArgumentError, DimensionMismatch, MethodError
+from .errors import ArgumentError, DefaultIntersectionFailureError, DimensionMismatch, ErrorException, MethodError
+from .manifolds import Geodesic, Planar, Spherical, best_manifold, resolve_manifold
 from .grids import (Grid, NESTED, RING, as_grid, cubed_sphere_points, healpix_cells,
                     flatten_tree, healpix_nest2ring, healpix_ring2nest, lonlat_points, lonlat_to_unit, points_to_cells,
                     polygons_to_cells, polygons_width, ring_full_polygons, ring_latitudes, ring_longitudes)
@@ -22,7 +23,8 @@ __all__ = [
     "Grid", "NESTED", "RING", "as_grid", "lonlat_points", "cubed_sphere_points", "healpix_cells",
     "healpix_nest2ring", "healpix_ring2nest", "points_to_cells", "lonlat_to_unit", "polygons_to_cells",
     "polygons_width", "flatten_tree", "ring_full_polygons", "ring_latitudes", "ring_longitudes",
-    "ArgumentError", "DimensionMismatch", "MethodError", "CRError", "EARTH_RADIUS", "LIB_PATH",
+    "ArgumentError", "DimensionMismatch", "MethodError", "ErrorException", "DefaultIntersectionFailureError",
+    "CRError", "EARTH_RADIUS", "LIB_PATH", "Planar", "Spherical", "Geodesic", "best_manifold", "resolve_manifold",
     "device_count", "version",
 ]
 

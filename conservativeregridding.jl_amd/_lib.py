@@ -41,7 +41,8 @@ class BuildStats(ctypes.Structure):
     _fields_ = [("n_dst", c_i64), ("n_src", c_i64), ("nnz", c_i64), ("n_candidates", c_i64),
                 ("ms_total", c_dbl), ("ms_bounds", c_dbl), ("ms_tree", c_dbl), ("ms_search", c_dbl),
                 ("ms_clip", c_dbl), ("ms_assemble", c_dbl), ("clip_with_dst", c_i32),
-                ("flipped_cells", c_i32), ("ms_host", c_dbl)]
+                ("flipped_cells", c_i32), ("ms_host", c_dbl), ("n_zero_area_cells", c_i64),
+                ("n_zero_cell_pairs", c_i64), ("n_nonfinite_pairs", c_i64), ("n_nonfinite_cells", c_i64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -57,6 +58,7 @@ class EsmfAttrs(ctypes.Structure):
 SIGNATURES = {
     "cr_last_error": (ctypes.c_char_p, []),
     "cr_version": (ctypes.c_char_p, []),
+    "cr_last_failed_pair": (c_i32, [_i64p, _i64p]),
     "cr_device_count": (c_i32, []),
     "cr_gen_lonlat_points": (c_i32, [_dp, c_i64, c_i64, c_dbl, c_dbl, c_dbl, c_dbl]),
     "cr_gen_cubed_sphere_points": (c_i32, [_dp, c_i64]),
@@ -67,6 +69,8 @@ SIGNATURES = {
     "cr_points_to_cells": (c_i32, [_dp, c_i64, c_i64, c_i64, _dp]),
     "cr_polygons_width": (c_i32, [c_i64, _i64p, _i32p, c_i64, _i32p]),
     "cr_polygons_to_cells": (c_i32, [_dp, c_i64, _i64p, _i32p, c_i64, c_i32, _dp, _i32p]),
+    "cr_validate_cells": (c_i32, [_dp, c_i64, c_i32, c_i32, _i64p]),
+    "cr_validate_points": (c_i32, [_dp, c_i64, c_i64, c_i64, _i64p]),
     "cr_grid_from_cells": (c_i32, [ctypes.POINTER(c_vp), _dp, c_i64, c_i32, c_i32, c_vp]),
     "cr_grid_from_polygons": (c_i32, [ctypes.POINTER(c_vp), _dp, c_i64, _i64p, _i32p, c_i64, c_i32, c_vp]),
     "cr_grid_from_points": (c_i32, [ctypes.POINTER(c_vp), _dp, c_i64, c_i64, c_i64, c_vp]),
@@ -102,6 +106,7 @@ SIGNATURES = {
                           c_vp]),
     "cr_regrid_backward": (c_i32, [c_vp, c_vp, c_vp, c_i32, c_i32, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64, c_i64,
                                    c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp]),
+    "cr_regrid_prepare": (c_i32, [c_vp, c_i32, c_i32, c_i64, c_i64, c_i64, c_vp]),
     "cr_regrid_tune": (c_i32, [c_vp, c_i32, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, _i32p,
                                ctypes.POINTER(ctypes.c_float), c_vp]),
     "cr_regrid_variant_count": (c_i32, [c_i32]),

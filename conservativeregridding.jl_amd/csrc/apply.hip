@@ -205,6 +205,7 @@ __global__ void k_long_rows(const int64_t *__restrict__ rowptr, int64_t n, int32
 static void ensure_long_rows(const Csr &A, hipStream_t s) {
     std::lock_guard<std::mutex> lock(A.plan_mu);
     if (A.nlong >= 0) return;
+    CR_REQUIRE_NOT_CAPTURING(s, "the long-row list of this direction");
     int64_t nlong = 0;
     if (A.nrows) {
         DevBuf<int32_t> list(A.nrows, s);
@@ -297,7 +298,9 @@ constexpr F32Variant kF32Variants[] = {
     {true, 4, true, 8, 1, 0},    {true, 4, true, 16, 1, 0},   {true, 4, true, 1, 0, 0},
     {true, 4, true, 0, 0, 1},    {true, 2, true, 0, 0, 1},    {true, 4, true, 4, 0, 1},
     {true, 4, false, 0, 0, 1},   {true, 4, false, 4, 0, 1},   {true, 4, 2, 0, 0, 1},
-    {true, 4, 2, 4, 0, 1},       {true, 4, 2, 0, 0, 1, 1},    {true, 4, 2, 4, 0, 1, 1}};
+    {true, 4, 2, 4, 0, 1},       {true, 4, 2, 0, 0, 1, 1},    {true, 4, 2, 4, 0, 1, 1},
+    {true, 4, 2, 4, 1, 1},       {true, 4, 1, 4, 1, 1},       {true, 4, 2, 2, 1, 1},
+    {true, 4, 2, 4, 1, 1, 1}};
 constexpr int kNumVariants = sizeof(kF32Variants) / sizeof(kF32Variants[0]);
 constexpr int kDefaultF32Variant = 21;  // 1024-row tiles, 4 fields per stage, FP64 sums of FP32 A
 constexpr int kBlockF32Variant = 23;    // the same on the block-ordered plan
@@ -309,6 +312,7 @@ constexpr int kBlockF32Variant = 23;    // the same on the block-ordered plan
 static int default_f32_variant(const Csr &A, hipStream_t s) {
     int v = A.default_variant.load();
     if (v >= 0) return v;
+    CR_REQUIRE_NOT_CAPTURING(s, "the default FP32 kernel of this direction");
     v = kDefaultF32Variant;
     const TilePlan &P = tile_plan(A, kRounds, kGeomL, s, false);
     if (P.tile_cols > 0 && P.tile_lines * 32 > 3 * P.tile_cols) {
@@ -370,6 +374,36 @@ void apply(const cr_regridder *R, bool transpose, int dtype, int64_t nf, const v
                            normalize ? A.inv_area.p : nullptr, s);
     }
     CR_HIP(hipGetLastError());
+}
+
+// The per-direction set-up the apply above makes on its first call with these
+// arguments (default variant, tile plan or long-row list), without launching
+// the apply: afterwards such calls allocate nothing and never synchronise, so
+// they can be captured into a HIP graph.
+void prepare(const cr_regridder *R, bool transpose, int dtype, int64_t nf, int64_t sf, int64_t df,
+             hipStream_t s) {
+    const Csr &A = transpose ? R->bwd : R->fwd;
+    if (A.nrows == 0 || nf == 0) return;
+    if (dtype == CR_F64) {
+        const int v = A.f64_variant >= 0 ? A.f64_variant : kDefaultF64Variant;
+        if (v == 1) {
+            const TilePlan &P = tile_plan(A, kEven, kGeomS, s, false);
+            (void)P;
+        } else {
+            ensure_long_rows(A, s);
+        }
+    } else if (!(sf == 1 && df == 1 && nf >= 64)) {
+        int v = A.tuned_variant;
+        if (v < 0 || v >= kNumVariants) v = default_f32_variant(A, s);
+        const F32Variant &V = kF32Variants[v];
+        if (V.tiled) {
+            const TilePlan &P = tile_plan(A, kRounds, V.geom, s, V.blocks != 0);
+            (void)P;
+        } else {
+            ensure_long_rows(A, s);
+        }
+    }
+    CR_HIP(hipStreamSynchronize(s));
 }
 
 // Fold partners: y[partner] = y[primary] for every field (the mirror kernel of
@@ -454,6 +488,17 @@ cr_status cr_regrid(const cr_regridder *R, int32_t transpose, int32_t dtype, int
         CR_REQUIRE(nfields == 0 || (src && dst), CR_INVALID_ARGUMENT, "cr_regrid: null field pointer");
         apply(R, transpose != 0, dtype, nfields, src, src_cell_stride, src_field_stride, dst,
               dst_cell_stride, dst_field_stride, normalize != 0, as_stream(stream));
+    });
+}
+
+cr_status cr_regrid_prepare(const cr_regridder *R, int32_t transpose, int32_t dtype, int64_t nfields,
+                            int64_t src_field_stride, int64_t dst_field_stride, void *stream) {
+    return guard([&] {
+        CR_REQUIRE(R, CR_INVALID_ARGUMENT, "cr_regrid_prepare: null regridder");
+        CR_REQUIRE(dtype == CR_F64 || dtype == CR_F32, CR_UNSUPPORTED, "cr_regrid_prepare: dtype must be F64 or F32");
+        CR_REQUIRE(nfields >= 0 && src_field_stride >= 0 && dst_field_stride >= 0, CR_INVALID_ARGUMENT,
+                   "cr_regrid_prepare: bad sizes or strides");
+        prepare(R, transpose != 0, dtype, nfields, src_field_stride, dst_field_stride, as_stream(stream));
     });
 }
 

@@ -576,8 +576,14 @@ constexpr int kClipChunks = 8;
 // half of a tripolar fold row, ext/ConservativeRegriddingOceananigansExt.jl:65-115,
 // never a tree candidate there) intersects nothing: as a clip polygon its
 // zero-length edges would pass the whole subject.
+// The compaction counts the two other outcomes: kZeroCellPair marks a pair
+// dropped because one of its cells has zero area, and a non-finite area stays
+// as it is (the build then fails and names the pair).
+constexpr double kZeroCellPair = -1.0;
 __device__ __forceinline__ double kept_area(double a, double amin, double thr) {
-    return amin > 0.0 && a > thr * amin ? a : 0.0;
+    if (!(amin > 0.0)) return kZeroCellPair;
+    if (!isfinite(a)) return a;
+    return a > thr * amin ? a : 0.0;
 }
 
 template <int NS, int NC>
@@ -715,8 +721,11 @@ __global__ void __launch_bounds__(256) k_clip(const int32_t *__restrict__ cand_d
 }
 
 // Generic clip for convex clip polygons with more than 4 vertices: classic
-// buffered Sutherland-Hodgman in private memory.
-constexpr int kGenericMaxV = 96;
+// buffered Sutherland-Hodgman in private memory.  Each clip edge adds at most
+// one vertex to a convex subject's ring, so ns + nc <= 2 * kMaxCellVertices
+// vertices always fit; the overflow branch (a NaN area, which fails the build)
+// is a guard that cannot trigger.
+constexpr int kGenericMaxV = 2 * kMaxCellVertices;
 
 __device__ double clip_area_buffered(const double *clip, int nc, const double *subj, int ns) {
     d3 bufA[kGenericMaxV], bufB[kGenericMaxV];
@@ -730,7 +739,8 @@ __device__ double clip_area_buffered(const double *clip, int nc, const double *s
         int w = 0;
         d3 p = cur[m - 1];
         double sp = dot(n, p - a);
-        for (int i = 0; i < m && w < kGenericMaxV - 2; ++i) {
+        for (int i = 0; i < m; ++i) {
+            if (w > kGenericMaxV - 2) return __builtin_nan("");
             d3 q = cur[i];
             double sq = dot(n, q - a);
             if (sq >= 0.0) {
@@ -772,18 +782,53 @@ __global__ void k_clip_generic(const int32_t *__restrict__ cand_dst,
 // ---- wave64 ballot compaction of kept candidates --------------------------
 constexpr int kCompactBlock = 256;
 
+// Failure accounting of a build (one device block, zeroed per build):
+enum {
+    kAcctNonfinitePairs = 0,  // candidate pairs with a non-finite area
+    kAcctZeroCellPairs,       // candidate pairs dropped for a zero-area cell
+    kAcctFirstBadPair,        // lowest such candidate index (initialised to all ones)
+    kAcctZeroCells,           // zero-area cells, both grids
+    kAcctNonfiniteCells,      // non-finite cell areas, both grids
+    kAcctFirstBadCell,        // lowest such cell: destination index, or n_dst + source index
+    kAcctWords
+};
+
+// Zero-area and non-finite cell areas of one grid (base: 0 for the
+// destination, n_dst for the source in kAcctFirstBadCell).
+__global__ void k_count_cells(const double *__restrict__ a, int64_t n, int64_t base,
+                              unsigned long long *__restrict__ acct) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double v = c < n ? a[c] : 1.0;
+    const uint64_t mz = __ballot(v == 0.0), mb = __ballot(!isfinite(v));
+    if ((threadIdx.x & 63) == 0) {
+        if (mz) atomicAdd(&acct[kAcctZeroCells], (unsigned long long)__popcll(mz));
+        if (mb) {
+            atomicAdd(&acct[kAcctNonfiniteCells], (unsigned long long)__popcll(mb));
+            atomicMin(&acct[kAcctFirstBadCell], (unsigned long long)(base + c + __ffsll((long long)mb) - 1));
+        }
+    }
+}
+
 __global__ void __launch_bounds__(kCompactBlock) k_keep_count(const double *__restrict__ val,
                                                               int64_t n,
                                                               int64_t *__restrict__ block_cnt,
-                                                              uint64_t *__restrict__ keep_mask) {
+                                                              uint64_t *__restrict__ keep_mask,
+                                                              unsigned long long *__restrict__ acct) {
     __shared__ int wave_cnt[kCompactBlock / 64];
     int64_t c = (int64_t)blockIdx.x * kCompactBlock + threadIdx.x;
-    bool keep = c < n && val[c] > 0.0;
+    const double v = c < n ? val[c] : 0.0;
+    bool keep = v > 0.0;  // not kZeroCellPair, 0 or NaN
     uint64_t m = __ballot(keep);
+    const uint64_t mz = __ballot(v == kZeroCellPair), mb = __ballot(!isfinite(v));
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) {
         wave_cnt[wave] = __popcll(m);
         keep_mask[(int64_t)blockIdx.x * (kCompactBlock / 64) + wave] = m;
+        if (mz) atomicAdd(&acct[kAcctZeroCellPairs], (unsigned long long)__popcll(mz));
+        if (mb) {
+            atomicAdd(&acct[kAcctNonfinitePairs], (unsigned long long)__popcll(mb));
+            atomicMin(&acct[kAcctFirstBadPair], (unsigned long long)(c + __ffsll((long long)mb) - 1));
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -957,6 +1002,12 @@ void csr_transpose(const Csr &F, const int32_t *coo_row, Csr &Bk, Temp &T, hipSt
     CR_HIP(hipGetLastError());
 }
 
+static thread_local int64_t g_failed_dst = -1, g_failed_src = -1;
+void set_failed_pair(int64_t d, int64_t s) {
+    g_failed_dst = d;
+    g_failed_src = s;
+}
+
 void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
                      const cr_build_opts &o, hipStream_t s) {
     const int64_t nd = dst->ncells, ns = src->ncells;
@@ -976,6 +1027,13 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     }
     const int clip_nv = clip_is_dst ? dst->nv : src->nv;
 
+    // failure accounting (first-index words start at all ones for atomicMin)
+    DevBuf<unsigned long long> acct(kAcctWords, s);
+    CR_HIP(hipMemsetAsync(acct.p, 0, acct.bytes(), s));
+    CR_HIP(hipMemsetAsync(acct.p + kAcctFirstBadPair, 0xff, sizeof(unsigned long long), s));
+    CR_HIP(hipMemsetAsync(acct.p + kAcctFirstBadCell, 0xff, sizeof(unsigned long long), s));
+    unsigned long long h_acct[kAcctWords] = {};
+
     // 1. areas + caps
     DevBuf<double> a_dst, a_src;
     a_dst.alloc_kept(nd, s);
@@ -983,6 +1041,8 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     DevBuf<float4> cap_dst(nd, s), cap_src(ns, s);
     launch_cell_bounds(dst, a_dst.p, cap_dst.p, s);
     launch_cell_bounds(src, a_src.p, cap_src.p, s);
+    hipLaunchKernelGGL(k_count_cells, dim3(grid_for(nd, B)), dim3(B), 0, s, a_dst.p, nd, int64_t(0), acct.p);
+    hipLaunchKernelGGL(k_count_cells, dim3(grid_for(ns, B)), dim3(B), 0, s, a_src.p, ns, nd, acct.p);
     CR_HIP(hipGetLastError());
     tm.mark();  // 1
 
@@ -1030,7 +1090,15 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     }
     int64_t ncand = 0;
     CR_HIP(hipMemcpyAsync(&ncand, cand_off.p + nd, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    CR_HIP(hipMemcpyAsync(h_acct, acct.p, sizeof(h_acct), hipMemcpyDeviceToHost, s));
     CR_HIP(hipStreamSynchronize(s));
+    if (h_acct[kAcctNonfiniteCells]) {
+        const int64_t c = (int64_t)h_acct[kAcctFirstBadCell];
+        set_failed_pair(c < nd ? c : -1, c < nd ? -1 : c - nd);
+        throw Error{CR_DEGENERATE_GEOMETRY,
+                    std::to_string(h_acct[kAcctNonfiniteCells]) + " cell(s) have a non-finite area, the first " +
+                        (c < nd ? "destination cell " + std::to_string(c) : "source cell " + std::to_string(c - nd))};
+    }
     CR_REQUIRE(ncand < (int64_t(1) << 31), CR_UNSUPPORTED, "more than 2^31 candidate pairs");
     DevBuf<int32_t> cand_src(ncand, s), cand_dst(ncand, s);
     if (ncand) {  // zero-fill: any slot the fill pass skipped is the valid pair (0, 0)
@@ -1074,12 +1142,25 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
         CR_HIP(hipMemsetAsync(blk_cnt.p, 0, blk_cnt.bytes(), s));
         if (ncand)
             hipLaunchKernelGGL(k_keep_count, dim3((unsigned)nblk), dim3(kCompactBlock), 0, s, cval.p,
-                               ncand, blk_cnt.p, keep_mask.p);
+                               ncand, blk_cnt.p, keep_mask.p, acct.p);
         exclusive_sum(T, blk_cnt.p, blk.p, nblk + 1, s);
     }
     int64_t nnz = 0;
     CR_HIP(hipMemcpyAsync(&nnz, blk.p + nblk, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    CR_HIP(hipMemcpyAsync(h_acct, acct.p, sizeof(h_acct), hipMemcpyDeviceToHost, s));
     CR_HIP(hipStreamSynchronize(s));
+    if (h_acct[kAcctNonfinitePairs]) {
+        // the reference's DefaultIntersectionFailureError (regridder.jl:64-73): name the pair
+        const int64_t c = (int64_t)h_acct[kAcctFirstBadPair];
+        int32_t pd = -1, ps = -1;
+        CR_HIP(hipMemcpyAsync(&pd, cand_dst.p + c, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        CR_HIP(hipMemcpyAsync(&ps, cand_src.p + c, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        CR_HIP(hipStreamSynchronize(s));
+        set_failed_pair(pd, ps);
+        throw Error{CR_DEGENERATE_GEOMETRY, std::to_string(h_acct[kAcctNonfinitePairs]) +
+                                                " intersection(s) have a non-finite area, the first of source cell " +
+                                                std::to_string(ps) + " and destination cell " + std::to_string(pd)};
+    }
 
     Csr &F = R->fwd;
     F.nrows = nd; F.ncols = ns; F.nnz = nnz;
@@ -1127,6 +1208,10 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     st.ms_total = tm.ms(0, 5);
     st.clip_with_dst = clip_is_dst;
     st.flipped_cells = dst->flipped + src->flipped;
+    st.n_zero_area_cells = (int64_t)h_acct[kAcctZeroCells];
+    st.n_zero_cell_pairs = (int64_t)h_acct[kAcctZeroCellPairs];
+    st.n_nonfinite_pairs = 0;
+    st.n_nonfinite_cells = 0;
 }
 
 // normalize!(R): every stored value and both area vectors divided by max(A).
@@ -1182,6 +1267,7 @@ cr_status cr_regridder_build(cr_regridder **R, const cr_grid *dst, const cr_grid
         CR_REQUIRE(o.threshold_rel >= 0.0, CR_INVALID_ARGUMENT, "threshold_rel must be >= 0");
         CR_REQUIRE(o.radius > 0.0, CR_INVALID_ARGUMENT, "radius must be > 0");
         CR_REQUIRE(o.clip_side >= 0 && o.clip_side <= 2, CR_INVALID_ARGUMENT, "bad clip_side");
+        set_failed_pair(-1, -1);
         check_device();
         const auto t0 = std::chrono::steady_clock::now();
         hipStream_t s = as_stream(stream);
@@ -1205,6 +1291,14 @@ cr_status cr_regridder_build(cr_regridder **R, const cr_grid *dst, const cr_grid
             throw;
         }
         *R = r;
+    });
+}
+
+cr_status cr_last_failed_pair(int64_t *dst_cell, int64_t *src_cell) {
+    return guard([&] {
+        CR_REQUIRE(dst_cell && src_cell, CR_INVALID_ARGUMENT, "cr_last_failed_pair: null argument");
+        *dst_cell = g_failed_dst;
+        *src_cell = g_failed_src;
     });
 }
 

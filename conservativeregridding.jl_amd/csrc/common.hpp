@@ -52,6 +52,23 @@ cr_status guard(F &&f) {
     }
 }
 
+// True while `s` is being captured into a HIP graph.  Lazy per-regridder set-up
+// (tile plans, long-row lists, the default variant) allocates, copies to the
+// host and synchronises: none of that may run inside a capture.
+inline bool stream_capturing(hipStream_t s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess) {
+        (void)hipGetLastError();  // e.g. the legacy null stream during a global capture
+        return true;
+    }
+    return st != hipStreamCaptureStatusNone;
+}
+
+#define CR_REQUIRE_NOT_CAPTURING(s, what)                                                                \
+    CR_REQUIRE(!::cr::stream_capturing(s), CR_UNSUPPORTED,                                               \
+               std::string(what) + " is not made yet and cannot be made during stream capture: call "   \
+                                   "cr_regrid_prepare (or one cr_regrid) with the same arguments first")
+
 // Device memory.  DevBuf(n) uses hipMalloc (long-lived buffers owned by a
 // handle); DevBuf(n, stream) takes stream-ordered memory from the device's
 // default pool (build temporaries), which keeps freed blocks for the next

@@ -1,5 +1,6 @@
 // grids.hip -- grid generators (host entry points) and device grid creation.
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <thread>
 #include <vector>
@@ -135,6 +136,96 @@ void parallel_for(int64_t n, F &&f) {
         });
     }
     for (auto &x : th) x.join();
+}
+
+// Host checks of a caller's cells before they reach the device.  Every
+// coordinate finite and every vertex a unit vector: the points of the
+// spherical manifold (UnitSphericalPoint, src/trees/interfaces.jl:125-126;
+// CR_INVALID_ARGUMENT otherwise).  No edge between antipodal points: its great
+// circle is undefined, so the clip and the cell area are not
+// (CR_DEGENERATE_GEOMETRY).  For a grid declared convex, every cell convex: the
+// Sutherland-Hodgman clip needs a convex clip polygon (regridder.jl:96-103;
+// CR_DEGENERATE_GEOMETRY).  A cell whose vertices all coincide is legal (a
+// zero-area ghost that intersects nothing) and is counted in *collapsed.
+// get(c, k) returns vertex k of cell c.
+constexpr double kUnitTol = 1e-10;      // | |v|^2 - 1 | of a unit vector
+constexpr double kConvexTol = 1e-12;    // sine of the angle a vertex may lie outside an edge
+constexpr double kCoincident2 = 1e-28;  // squared chord of coincident vertices
+
+// first(x, c): x = min(x, c), so that a report names the lowest offending cell
+inline void first(std::atomic<int64_t> &x, int64_t c) {
+    int64_t cur = x.load();
+    while (c < cur && !x.compare_exchange_weak(cur, c)) {
+    }
+}
+
+template <class Get>
+void validate_cells(int64_t ncells, int nv, bool convex, Get get, const char *what, int64_t *collapsed) {
+    constexpr int64_t kNone = INT64_MAX;
+    std::atomic<int64_t> bad_unit{kNone}, bad_anti{kNone}, bad_convex{kNone}, ncoll{0};
+    parallel_for(ncells, [&](int64_t c) {
+        bool any_edge = false;
+        double smin = 0.0, smax = 0.0;
+        for (int k = 0; k < nv; ++k) {
+            const double *p = get(c, k);
+            const d3 a = load3(p);
+            const double r2 = dot(a, a);
+            if (!(fabs(r2 - 1.0) <= kUnitTol)) {  // also false for NaN or infinite coordinates
+                first(bad_unit, c);
+                return;
+            }
+        }
+        for (int k = 0; k < nv; ++k) {
+            const d3 a = load3(get(c, k)), b = load3(get(c, (k + 1) % nv));
+            const d3 e = b - a;
+            if (dot(e, e) < kCoincident2) continue;  // a repeated vertex (pole corner, padding)
+            any_edge = true;
+            const d3 s = a + b;
+            if (dot(s, s) < kCoincident2) {
+                first(bad_anti, c);
+                return;
+            }
+            if (!convex) continue;
+            const d3 n = cross(a, e);
+            const double nn = dot(n, n);
+            if (nn < kDegenerateEdge2) continue;
+            const double inv = 1.0 / sqrt(nn);
+            for (int j = 0; j < nv; ++j) {
+                const double sj = dot(n, load3(get(c, j)) - a) * inv;
+                smin = std::min(smin, sj);
+                smax = std::max(smax, sj);
+            }
+        }
+        if (!any_edge) ncoll.fetch_add(1);
+        // convex: every vertex on one side of every edge (either orientation;
+        // cells are turned counter-clockwise on the device)
+        if (convex && smin < -kConvexTol && smax > kConvexTol) first(bad_convex, c);
+    });
+    CR_REQUIRE(bad_unit.load() == kNone, CR_INVALID_ARGUMENT,
+               std::string(what) + ": cell " + std::to_string(bad_unit.load()) +
+                   " has a vertex that is not a finite unit vector (grids are on the unit sphere)");
+    CR_REQUIRE(bad_anti.load() == kNone, CR_DEGENERATE_GEOMETRY,
+               std::string(what) + ": cell " + std::to_string(bad_anti.load()) +
+                   " has an edge between antipodal points (its great circle is undefined)");
+    CR_REQUIRE(bad_convex.load() == kNone, CR_DEGENERATE_GEOMETRY,
+               std::string(what) + ": cell " + std::to_string(bad_convex.load()) +
+                   " is not convex but the grid is declared convex (the clip needs a convex side)");
+    if (collapsed) *collapsed = ncoll.load();
+}
+
+// The cells of nblocks point matrices ((ni+1) x (nj+1), i fastest), in the
+// CellBasedGrid vertex order of src/trees/grids.jl:71-84.
+void validate_point_blocks(const double *pts, int64_t nblocks, int64_t ni, int64_t nj, const char *what,
+                           int64_t *collapsed) {
+    const int64_t W = ni + 1, per = ni * nj;
+    validate_cells(
+        nblocks * per, 4, true,
+        [&](int64_t c, int k) {
+            const int64_t b = c / per, r = c - b * per, i = r % ni, j = r / ni;
+            const int64_t di = (k == 1 || k == 2) ? 1 : 0, dj = (k >= 2) ? 1 : 0;
+            return pts + 3 * (b * W * (nj + 1) + (i + di) + (j + dj) * W);
+        },
+        what, collapsed);
 }
 
 // Vertex count of polygon c of a ragged list; a ring whose last index repeats
@@ -291,6 +382,9 @@ cr_status cr_grid_from_cells(cr_grid **grid, const double *cells, int64_t ncells
         CR_REQUIRE(grid && cells && ncells > 0 && nv >= 3 && nv <= kMaxCellVertices, CR_INVALID_ARGUMENT,
                    "cr_grid_from_cells: need ncells > 0 and 3 <= nv <= 64");
         CR_REQUIRE(ncells < (int64_t(1) << 31), CR_UNSUPPORTED, "more than 2^31-1 cells");
+        validate_cells(
+            ncells, nv, convex != 0, [&](int64_t c, int k) { return cells + 3 * (c * nv + k); },
+            "cr_grid_from_cells", nullptr);
         check_device();
         hipStream_t s = as_stream(stream);
         auto *g = new cr_grid();
@@ -343,11 +437,15 @@ cr_status cr_grid_from_polygons(cr_grid **grid, const double *xyz, int64_t nvert
                    "cr_grid_from_polygons: bad arguments");
         CR_REQUIRE(ncells < (int64_t(1) << 31) && nverts < (int64_t(1) << 31), CR_UNSUPPORTED,
                    "more than 2^31-1 cells or vertices");
-        check_device();
         const int32_t nv = polygons_width(nverts, offsets, verts, ncells);
         std::vector<double> cells((size_t)ncells * nv * 3);
         std::vector<int32_t> vidx((size_t)ncells * nv);
         polygons_fill(xyz, offsets, verts, ncells, nv, cells.data(), vidx.data());
+        const double *cp = cells.data();
+        validate_cells(
+            ncells, nv, convex != 0, [&](int64_t c, int k) { return cp + 3 * (c * nv + k); },
+            "cr_grid_from_polygons", nullptr);
+        check_device();
         hipStream_t s = as_stream(stream);
         auto *g = new cr_grid();
         try {
@@ -376,6 +474,7 @@ cr_status cr_grid_from_points(cr_grid **grid, const double *pts, int64_t nblocks
         CR_REQUIRE(grid && pts && nblocks > 0 && ni > 0 && nj > 0, CR_INVALID_ARGUMENT,
                    "cr_grid_from_points: bad arguments");
         CR_REQUIRE(nblocks * ni * nj < (int64_t(1) << 31), CR_UNSUPPORTED, "more than 2^31-1 cells");
+        validate_point_blocks(pts, nblocks, ni, nj, "cr_grid_from_points", nullptr);
         check_device();
         hipStream_t s = as_stream(stream);
         int64_t npts = nblocks * (ni + 1) * (nj + 1);
@@ -407,6 +506,7 @@ cr_status cr_grid_tripolar(cr_grid **grid, const double *pts, int64_t nx, int64_
                    "cr_grid_tripolar: need nx >= 2 and ny >= 2");
         CR_REQUIRE(nx % 2 == 0, CR_INVALID_ARGUMENT, "a RightCenterFolded grid needs an even number of cells in x");
         CR_REQUIRE(nx * ny < (int64_t(1) << 31), CR_UNSUPPORTED, "more than 2^31-1 cells");
+        validate_point_blocks(pts, 1, nx, ny, "cr_grid_tripolar", nullptr);
         check_device();
         hipStream_t s = as_stream(stream);
         const int64_t npts = (nx + 1) * (ny + 1);
@@ -435,6 +535,23 @@ cr_status cr_grid_tripolar(cr_grid **grid, const double *pts, int64_t nx, int64_
             throw;
         }
         *grid = g;
+    });
+}
+
+cr_status cr_validate_cells(const double *cells, int64_t ncells, int32_t nv, int32_t convex, int64_t *collapsed) {
+    return guard([&] {
+        CR_REQUIRE(cells && ncells > 0 && nv >= 3 && nv <= kMaxCellVertices, CR_INVALID_ARGUMENT,
+                   "cr_validate_cells: need ncells > 0 and 3 <= nv <= 64");
+        validate_cells(
+            ncells, nv, convex != 0, [&](int64_t c, int k) { return cells + 3 * (c * nv + k); },
+            "cr_validate_cells", collapsed);
+    });
+}
+
+cr_status cr_validate_points(const double *pts, int64_t nblocks, int64_t ni, int64_t nj, int64_t *collapsed) {
+    return guard([&] {
+        CR_REQUIRE(pts && nblocks > 0 && ni > 0 && nj > 0, CR_INVALID_ARGUMENT, "cr_validate_points: bad arguments");
+        validate_point_blocks(pts, nblocks, ni, nj, "cr_validate_points", collapsed);
     });
 }
 

@@ -740,6 +740,7 @@ const TilePlan &tile_plan(const Csr &A, int style, int geom, hipStream_t s, bool
     std::lock_guard<std::mutex> lock(A.plan_mu);
     auto &slot = A.tiles[style][geom][blocks ? 1 : 0];
     if (slot) return *slot;
+    CR_REQUIRE_NOT_CAPTURING(s, "the tile plan of this direction");
     auto P = std::make_shared<TilePlan>();
     P->geom = geom;
     if (geom == kGeomL) {

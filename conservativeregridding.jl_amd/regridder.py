@@ -35,11 +35,9 @@ import math
 import numpy as np
 
 from . import _lib as L
-from .errors import ArgumentError, DimensionMismatch, MethodError
+from .errors import ArgumentError, DefaultIntersectionFailureError, DimensionMismatch, MethodError
 from .grids import Grid, as_grid
-
-#: GeometryOpsCore's default ``Spherical()`` radius (mean Earth radius, metres)
-EARTH_RADIUS = 6371008.8
+from .manifolds import EARTH_RADIUS, Manifold, check_buildable, on_sphere, resolve_manifold
 
 
 class _Core:
@@ -89,47 +87,74 @@ class _Core:
             self.h = ctypes.c_void_p()
 
 
+def _raise_intersection_failure(dg: Grid, sg: Grid):
+    """DefaultIntersectionFailureError(p1 = source cell, p2 = destination
+    cell, e) for a build that failed on a non-finite area (regridder.jl:64-73)."""
+    msg = L.last_error()
+    d, s_ = ctypes.c_int64(-1), ctypes.c_int64(-1)
+    L.check(L.lib.cr_last_failed_pair(ctypes.byref(d), ctypes.byref(s_)), "last_failed_pair")
+    p1 = sg.cells()[s_.value] if s_.value >= 0 else None
+    p2 = dg.cells()[d.value] if d.value >= 0 else None
+    raise DefaultIntersectionFailureError(p1, p2, L.CRError(L.CR_DEGENERATE_GEOMETRY, msg))
+
+
 class Regridder:
     """Conservative regridder from ``src`` cells to ``dst`` cells (rows = dst).
 
     ``Regridder(dst, src; normalize=false, intersection_operator, threaded)`` of
-    src/regridder/regridder.jl:125-163.  ``dst``/``src`` are :class:`Grid` objects
-    or unit-vector point matrices ``(ni+1, nj+1, 3)``.  Extra keywords:
+    src/regridder/regridder.jl:125-163.  ``dst``/``src`` are :class:`Grid` objects,
+    unit-vector point matrices ``(ni+1, nj+1, 3)``, objects with the Trees
+    interface, or planar ``(lon_edges, lat_edges)`` tuples in degrees.  The
+    manifold is resolved as ``Regridder(dst, src)`` does (regridder.jl:105-123):
+    a planar grid next to a spherical one is promoted to ``Spherical()``, other
+    mismatches raise ``ErrorException``, and two planar grids raise
+    ``MethodError`` (planar clipping is not on the device).  A pair whose
+    intersection area is not finite raises ``DefaultIntersectionFailureError``
+    with the two cells.  Extra keywords:
 
     * ``threshold_rel`` -- pairs with area <= threshold_rel * min(cell areas) are
       dropped (default 1e-14; 0 keeps every positive area like the reference)
-    * ``radius`` -- sphere radius (areas in radius^2 units)
+    * ``manifold`` -- build on this manifold (``Regridder(manifold, dst, src)``)
+    * ``radius`` -- sphere radius (areas in radius^2 units); default: the
+      manifold's (``Spherical().radius`` = 6 371 008.8)
     * ``clip_side`` -- "auto", "dst" or "src": which side's (convex) cells clip
     * ``triangle_quad_degree`` -- Dunavant rule of the spectral-element weights
       when one side is a :class:`SpectralElementSpace` (default 2 (nq - 1))
     """
 
     def __init__(self, dst, src, *, normalize: bool = False, intersection_operator=None,
-                 threaded=True, threshold_rel: float = 1e-14, radius: float = EARTH_RADIUS,
-                 clip_side: str = "auto", triangle_quad_degree=None, stream=None, _core=None,
-                 _transposed=False):
+                 threaded=True, threshold_rel: float = 1e-14, radius: float | None = None,
+                 manifold: Manifold | None = None, clip_side: str = "auto", triangle_quad_degree=None,
+                 stream=None, _core=None, _transposed=False):
         if _core is not None:
             self._core, self._t = _core, _transposed
             return
         if intersection_operator is not None:
             raise MethodError("a custom intersection_operator runs per pair on the CPU; the device build "
                               "computes spherical clip areas only")
+        from .se import SpectralElementSpace, build_se_regridder
+        is_se = isinstance(dst, SpectralElementSpace) or isinstance(src, SpectralElementSpace)
+        if manifold is None and not is_se:
+            # Regridder(dst, src): best_manifold of each side, planar promoted (regridder.jl:105-123)
+            manifold = resolve_manifold(dst, src)
+        m_radius = check_buildable(manifold) if manifold is not None else EARTH_RADIUS
         opts = L.BuildOpts()
         L.lib.cr_build_opts_default(ctypes.byref(opts))
         opts.threshold_rel = float(threshold_rel)
-        opts.radius = float(radius)
+        opts.radius = float(m_radius if radius is None else radius)
         opts.normalize = 1 if normalize else 0
         opts.clip_side = {"auto": 0, "dst": 1, "src": 2}[clip_side]
-        from .se import SpectralElementSpace, build_se_regridder
-        if isinstance(dst, SpectralElementSpace) or isinstance(src, SpectralElementSpace):
+        if is_se:
             # the ClimaCore extension's constructors (ext/ConservativeRegriddingClimaCoreExt.jl:416-426, 489-508)
             h, self.dst_shape, self.src_shape = build_se_regridder(dst, src, opts, triangle_quad_degree, stream)
             self._core, self._t = _Core(h), False
             return
-        dg, sg = as_grid(dst), as_grid(src)
+        dg, sg = as_grid(on_sphere(dst)), as_grid(on_sphere(src))
         h = ctypes.c_void_p()
-        L.check(L.lib.cr_regridder_build(ctypes.byref(h), dg._h, sg._h, ctypes.byref(opts), _stream(stream)),
-                "Regridder")
+        st = L.lib.cr_regridder_build(ctypes.byref(h), dg._h, sg._h, ctypes.byref(opts), _stream(stream))
+        if st == L.CR_DEGENERATE_GEOMETRY:
+            _raise_intersection_failure(dg, sg)
+        L.check(st, "Regridder")
         self._core, self._t = _Core(h.value), False
         self._core.nfold = (dg.nfold, sg.nfold)
         self.dst_shape, self.src_shape = dg.shape, sg.shape
@@ -415,6 +440,10 @@ def regrid_plan(dst_field, R: Regridder, src_field, *, dims: int = 1, normalize:
     if outer:
         raise MethodError("regrid_plan needs field axes that flatten into one stride")
     code = _dtype_code(src_field.dtype)
+    # the per-direction set-up (default kernel, tile plan): afterwards the
+    # launch allocates nothing and never synchronises (capturable)
+    L.check(L.lib.cr_regrid_prepare(R._core.h, 1 if R._t else 0, code, nf, sf, df,
+                                    torch.cuda.current_stream().cuda_stream), "regrid_prepare")
     args = (R._core.h, 1 if R._t else 0, code, nf, src_field.data_ptr(), src_field.stride(dim), sf,
             dst_field.data_ptr(), dst_field.stride(dim), df, 1 if normalize else 0)
     fn = L.lib.cr_regrid

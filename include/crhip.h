@@ -75,10 +75,24 @@ typedef struct {
     int32_t clip_with_dst; /* 1: dst cells were the clip polygons */
     int32_t flipped_cells; /* cells re-oriented to CCW at grid creation */
     double ms_host;        /* host wall time of the whole build call */
+    /* Per-pair failure accounting (the reference's per-pair try/catch,
+     * regridder.jl:88-101, is disabled; here every case is counted): */
+    int64_t n_zero_area_cells;  /* cells of either grid with zero area (they intersect nothing) */
+    int64_t n_zero_cell_pairs;  /* candidate pairs dropped because one of their cells has zero area */
+    int64_t n_nonfinite_pairs;  /* candidate pairs whose intersection area is not finite: the build
+                                   fails with CR_DEGENERATE_GEOMETRY (cr_last_failed_pair) */
+    int64_t n_nonfinite_cells;  /* cells with a non-finite area: the build fails likewise */
 } cr_build_stats;
 
 /* ------------------------------------------------------------ library -- */
 const char *cr_last_error(void);
+/* The cells of the last build on this thread that failed with
+ * CR_DEGENERATE_GEOMETRY (-1 where a side is not involved, both -1 after a
+ * build that did not fail that way): the pair whose intersection area is not
+ * finite, or the one cell whose area is not.  The reference's
+ * DefaultIntersectionFailureError carries the two polygons
+ * (src/regridder/regridder.jl:64-73). */
+cr_status cr_last_failed_pair(int64_t *dst_cell, int64_t *src_cell);
 const char *cr_version(void);
 /* Number of visible HIP devices (0 on a host without a GPU; never fails). */
 int32_t cr_device_count(void);
@@ -129,6 +143,21 @@ cr_status cr_polygons_width(int64_t nverts, const int64_t *offsets, const int32_
 cr_status cr_polygons_to_cells(const double *xyz, int64_t nverts, const int64_t *offsets,
                                const int32_t *verts, int64_t ncells, int32_t nv, double *cells,
                                int32_t *vidx);
+
+/* Host checks every grid constructor below makes of a caller's cells before
+ * anything reaches the device (they need no GPU).  CR_INVALID_ARGUMENT: a
+ * coordinate is not finite or a vertex is not a unit vector (the points of the
+ * spherical manifold, UnitSphericalPoint: the reference promotes planar
+ * inputs to it in Regridder(dst, src), regridder.jl:105-123).
+ * CR_DEGENERATE_GEOMETRY: an edge joins antipodal points (no great circle),
+ * or, with convex=1, a cell is not convex (the Sutherland-Hodgman clip of
+ * regridder.jl:96-103 needs a convex clip polygon).  Cells whose vertices all
+ * coincide are legal zero-area cells that intersect nothing; *collapsed (may
+ * be NULL) receives their number.  The message names the lowest offending
+ * cell.  cr_validate_points checks the cells of point blocks (layout of
+ * cr_grid_from_points, always convex). */
+cr_status cr_validate_cells(const double *cells, int64_t ncells, int32_t nv, int32_t convex, int64_t *collapsed);
+cr_status cr_validate_points(const double *pts, int64_t nblocks, int64_t ni, int64_t nj, int64_t *collapsed);
 
 /* ------------------------------------------------------ device grids -- */
 /* Upload a host polygon table.  `convex` declares every cell convex. */
@@ -210,15 +239,34 @@ cr_status cr_mirror_fold_partners_adjoint(const cr_regridder *R, int32_t transpo
  * is at ptr[c*cell_stride + f*field_stride].  cells-fastest ([n,64,4] with
  * dims=1) is cell_stride=1, field_stride=n; fields-fastest is cell_stride=k,
  * field_stride=1.  normalize=1 divides by the destination areas
- * (finalize_regridding!, regrid.jl:104-118).  CR_F64 fields accumulate in FP64;
- * CR_F32 fields accumulate in FP32 with Kahan-compensated row sums against the
- * FP64 A.  The forward and transpose paths are gathers (no atomics): results
- * are bitwise reproducible.
+ * (finalize_regridding!, regrid.jl:104-118).  CR_F64 fields: products and
+ * sums in FP64 (fused multiply-adds), a true division by the area.  CR_F32
+ * fields: FP32 in and out; the default kernel multiplies each FP32 value by A
+ * (see cr_regrid_set_variant for whether A is used in FP64 or rounded to FP32)
+ * and sums in FP64, adds a row's partial sums in FP64, scales by the FP64
+ * reciprocal area and rounds once; variants 5-7 and 18-19 instead sum in FP32
+ * with Kahan compensation.  Both directions are gathers with a fixed summation
+ * order (no atomics): for a fixed kernel variant the results are bitwise
+ * reproducible (cr_regrid_tune may pick another variant, whose last bits can
+ * differ).  The first call of a direction makes its per-regridder set-up (see
+ * cr_regrid_prepare), which allocates and synchronises.
  */
 cr_status cr_regrid(const cr_regridder *R, int32_t transpose, int32_t dtype, int64_t nfields,
                     const void *src, int64_t src_cell_stride, int64_t src_field_stride,
                     void *dst, int64_t dst_cell_stride, int64_t dst_field_stride,
                     int32_t normalize, void *stream);
+
+/*
+ * Make the per-direction set-up that cr_regrid makes on its first call with
+ * this dtype, field count and field strides (the default FP32 kernel of the
+ * direction, its tile plan or the long-row list), without applying anything
+ * (no reference counterpart: Julia's mul! has no set-up).  After it, such
+ * cr_regrid calls allocate nothing and never synchronise, so they can be
+ * captured into a HIP graph; a cr_regrid that would still need set-up during a
+ * capture fails with CR_UNSUPPORTED instead.  Synchronises the stream.
+ */
+cr_status cr_regrid_prepare(const cr_regridder *R, int32_t transpose, int32_t dtype, int64_t nfields,
+                            int64_t src_field_stride, int64_t dst_field_stride, void *stream);
 
 /*
  * Reverse mode of regrid!(y, R, x) (transpose=1: of regrid!(y, transpose(R), x)),

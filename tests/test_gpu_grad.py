@@ -174,3 +174,71 @@ def test_cell_table_gradients_sum_to_point_gradients(cr, setup):
     np.add.at(summed, corner, g_slots)
     scale = np.abs(g_pts).max()
     np.testing.assert_allclose(summed, g_pts, rtol=0, atol=1e-10 * scale)
+
+
+@pytest.fixture(scope="module")
+def c4_full(cr):
+    """The differentiation configuration at its size: jittered C96 -> 1 deg."""
+    P = jittered_cubed_sphere(cr, 96, seed=4, frac=0.05)
+    D = cr.lonlat_points(360, 180)
+    rng = np.random.default_rng(21)
+    return P, D, rng.random(6 * 96 * 96), rng.random(360 * 180)
+
+
+def test_c4_full_size_gradients(cr, orc, c4_full):
+    """C96 -> 1 deg: field gradient vs 2 A^T((d - t) ./ a_d) to 1e-12; vertex
+    gradients of 200 random interior source vertices vs the dual-number pair
+    assembly to 1e-9; central differences of the oracle-built loss on 10 of
+    them to 1e-5."""
+    _case("grad_c4_full", src="C96 jittered seed 4", dst="1deg", vertices=200, fd_vertices=10)
+    P, D, s, t = c4_full
+    src, dst = cr.Grid.from_points(P), cr.Grid.from_points(D)
+    R = cr.Regridder(dst, src, radius=1.0)
+    st = torch.tensor(s, device=DEV)
+    d = cr.regrid(R, st)
+    gy = 2.0 * (d - torch.tensor(t, device=DEV))
+    gx, _, g_src = cr.regrid_backward(R, dst, src, st, d, gy)
+    g_src = g_src.cpu().numpy()
+    L0, ref, d_ref = oracle_loss(cr, orc, P, D, s, t)
+    coef = 2.0 * (d_ref - t) / ref.a_dst
+    T = ref.transpose()
+    gx_ref = orc.spmv(T.rowptr, T.col, T.val, coef)
+    assert np.max(np.abs(gx.cpu().numpy() - gx_ref)) <= 1e-12 * np.max(np.abs(gx_ref))
+
+    scells, dcells = cr.points_to_cells(P), cr.points_to_cells(D)
+    ne = P.shape[1] - 1
+    rng = np.random.default_rng(12)
+    picks = set()
+    while len(picks) < 200:
+        picks.add((int(rng.integers(0, 6)), int(rng.integers(1, ne)), int(rng.integers(1, ne))))
+    picks = sorted(picks)
+    worst = 0.0
+    for f, i, j in picks:
+        v = f * (ne + 1) ** 2 + j * (ne + 1) + i
+        g = np.zeros(3)
+        for (ci, cj, slot) in ((i - 1, j - 1, 2), (i, j - 1, 3), (i, j, 0), (i - 1, j, 1)):
+            c = f * ne * ne + cj * ne + ci
+            for k in range(T.rowptr[c], T.rowptr[c + 1]):
+                row = T.col[k]
+                _, gsub, _ = orc.clip_area_grad(scells[c], dcells[row])
+                g += coef[row] * s[c] * gsub[slot]
+        err = np.max(np.abs(g_src[v] - g)) / max(np.abs(g).max(), 1e-300)
+        worst = max(worst, err)
+        np.testing.assert_allclose(g_src[v], g, rtol=1e-9, atol=1e-12 * np.abs(g).max())
+    print(f"c4 dual-number assembly: 200 vertices, worst rel err {worst:.2e}", flush=True)
+
+    h = 1e-6
+    worst = 0.0
+    for f, i, j in picks[::20]:
+        v = f * (ne + 1) ** 2 + j * (ne + 1) + i
+        for k in range(3):
+            Pp, Pm = P.copy(), P.copy()
+            Pp[f, i, j, k] += h
+            Pm[f, i, j, k] -= h
+            Pp[f, i, j] /= np.linalg.norm(Pp[f, i, j])
+            Pm[f, i, j] /= np.linalg.norm(Pm[f, i, j])
+            fd = (oracle_loss(cr, orc, Pp, D, s, t)[0] - oracle_loss(cr, orc, Pm, D, s, t)[0]) / (2 * h)
+            scale = max(abs(fd), np.abs(g_src[v]).max(), 1e-3)
+            worst = max(worst, abs(g_src[v, k] - fd) / scale)
+    print(f"c4 central differences: 10 vertices, worst rel err {worst:.2e}", flush=True)
+    assert worst < 1e-5, worst
