@@ -10,7 +10,7 @@ from .manifolds import Geodesic, Planar, Spherical, best_manifold, resolve_manif
 from .grids import (Grid, NESTED, RING, as_grid, cubed_sphere_points, healpix_cells,
                     flatten_tree, healpix_nest2ring, healpix_ring2nest, lonlat_points, lonlat_to_unit, points_to_cells,
                     polygons_to_cells, polygons_width, ring_full_polygons, ring_latitudes, ring_longitudes)
-from .esmf import save_esmf_weights, write_esmf_arrays
+from .esmf import read_esmf_weights, regridder_from_esmf, save_esmf_weights, write_esmf_arrays
 from .se import SpectralElementSpace, se_moments
 from .regridder import (EARTH_RADIUS, Regridder, areas, normalize_, regrid, regrid_, regrid_plan,
                         regrid_tune, regrid_backward, regrid_variants, regrid_set_variant,
@@ -18,7 +18,7 @@ from .regridder import (EARTH_RADIUS, Regridder, areas, normalize_, regrid, regr
 
 __all__ = [
     "Regridder", "regrid_", "regrid", "regrid_plan", "regrid_tune", "regrid_backward", "regrid_variants",
-    "regrid_set_variant", "transpose", "normalize_", "areas", "size", "save_esmf_weights", "write_esmf_arrays",
+    "regrid_set_variant", "transpose", "normalize_", "areas", "size", "save_esmf_weights", "write_esmf_arrays", "read_esmf_weights", "regridder_from_esmf",
     "SpectralElementSpace", "se_moments",
     "Grid", "NESTED", "RING", "as_grid", "lonlat_points", "cubed_sphere_points", "healpix_cells",
     "healpix_nest2ring", "healpix_ring2nest", "points_to_cells", "lonlat_to_unit", "polygons_to_cells",

@@ -51,7 +51,8 @@ class BuildStats(ctypes.Structure):
 class EsmfAttrs(ctypes.Structure):
     _fields_ = [("src_grid_name", ctypes.c_char_p), ("dst_grid_name", ctypes.c_char_p),
                 ("created_at", ctypes.c_char_p), ("src_shape", ctypes.POINTER(c_i64)), ("src_ndim", c_i32),
-                ("dst_ndim", c_i32), ("dst_shape", ctypes.POINTER(c_i64))]
+                ("dst_ndim", c_i32), ("dst_shape", ctypes.POINTER(c_i64)), ("cdf_version", c_i32),
+                ("reserved", c_i32)]
 
 
 # name -> (restype, argtypes); exactly the entry points of include/crhip.h
@@ -111,8 +112,11 @@ SIGNATURES = {
                                ctypes.POINTER(ctypes.c_float), c_vp]),
     "cr_regrid_variant_count": (c_i32, [c_i32]),
     "cr_regridder_write_esmf": (c_i32, [c_vp, c_i32, ctypes.c_char_p, ctypes.POINTER(EsmfAttrs)]),
-    "cr_esmf_write": (c_i32, [ctypes.c_char_p, c_i64, _dp, _i32p, _i32p, c_i64, _dp, _dp, c_i64, _dp, _dp,
+    "cr_esmf_write": (c_i32, [ctypes.c_char_p, c_i64, _dp, _i64p, _i64p, c_i64, _dp, _dp, c_i64, _dp, _dp,
                               ctypes.POINTER(EsmfAttrs)]),
+    "cr_esmf_read_sizes": (c_i32, [ctypes.c_char_p, _i64p, _i64p, _i64p]),
+    "cr_esmf_read": (c_i32, [ctypes.c_char_p, _dp, _i64p, _i64p, _dp, _dp, _dp, _dp]),
+    "cr_regridder_from_esmf": (c_i32, [ctypes.POINTER(c_vp), ctypes.c_char_p, c_vp]),
     "cr_regrid_set_variant": (c_i32, [c_vp, c_i32, c_i32, c_i32]),
     "cr_regrid_tile_plan": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp]),
     "cr_regrid_copy_tile_plan": (c_i32, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,

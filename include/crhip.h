@@ -356,10 +356,12 @@ cr_status cr_regrid_copy_tile_plan(const cr_regridder *R, int32_t transpose, int
  * an ESMF offline-weights file with dimensions n_s, n_a, n_b, variables S =
  * A ./ dst_areas[row], row, col (1-based, column-major order of A), frac_a,
  * frac_b, area_a, area_b and the same global attributes.  Written as netCDF
- * classic with 64-bit offsets (CDF-2); row, col and the grid shapes are 32-bit
- * integers (the classic format has no 64-bit integer type).  Strings and
- * shapes in cr_esmf_attrs may be NULL / 0 (defaults "source", "destination";
- * created_at and the shapes are then omitted, as in the reference).
+ * classic: cdf_version 0 or 5 = CDF-5 (64-bit data, the default), with row,
+ * col and the grid shapes as 64-bit integers like the reference's Int64
+ * (findnz of a SparseMatrixCSC{Float64,Int64}); 2 = CDF-2 (64-bit offsets,
+ * 32-bit row/col, for readers without CDF-5).  Strings and shapes in
+ * cr_esmf_attrs may be NULL / 0 (defaults "source", "destination"; created_at
+ * and the shapes are then omitted, as in the reference).
  */
 typedef struct {
     const char *src_grid_name;
@@ -369,6 +371,8 @@ typedef struct {
     int32_t src_ndim;
     int32_t dst_ndim;
     const int64_t *dst_shape;
+    int32_t cdf_version; /* 0 or 5: CDF-5; 2: CDF-2 */
+    int32_t reserved;
 } cr_esmf_attrs;
 /* The file of this direction of R (transpose=1: transpose(R)); host work only
  * (synchronous copies of A and the areas). */
@@ -376,10 +380,25 @@ cr_status cr_regridder_write_esmf(const cr_regridder *R, int32_t transpose, cons
                                   const cr_esmf_attrs *attrs);
 /* The same file from host arrays (S, row, col of length n_s; frac_a, area_a of
  * n_a; frac_b, area_b of n_b); needs no GPU. */
-cr_status cr_esmf_write(const char *path, int64_t n_s, const double *S, const int32_t *row,
-                        const int32_t *col, int64_t n_a, const double *frac_a, const double *area_a,
+cr_status cr_esmf_write(const char *path, int64_t n_s, const double *S, const int64_t *row,
+                        const int64_t *col, int64_t n_a, const double *frac_a, const double *area_a,
                         int64_t n_b, const double *frac_b, const double *area_b,
                         const cr_esmf_attrs *attrs);
+/*
+ * Reading a weights file back (no reference counterpart: the reference only
+ * writes; reading lets a build be reused).  Any netCDF classic file (CDF-1,
+ * CDF-2 or CDF-5) with the dimensions n_s, n_a, n_b and 1-D variables of those
+ * names is accepted, integer or floating types converted.  cr_esmf_read_sizes
+ * gives the dimensions; cr_esmf_read fills the host arrays that are not NULL
+ * (S, row, col: n_s; frac_a, area_a: n_a; frac_b, area_b: n_b); both need no
+ * GPU.  cr_regridder_from_esmf makes a device regridder of it: A_ij =
+ * S_k * area_b[i] with i = row_k - 1, j = col_k - 1, dst_areas = area_b,
+ * src_areas = area_a (as cr_regridder_from_csr: no vertex gradients).
+ */
+cr_status cr_esmf_read_sizes(const char *path, int64_t *n_s, int64_t *n_a, int64_t *n_b);
+cr_status cr_esmf_read(const char *path, double *S, int64_t *row, int64_t *col, double *frac_a,
+                       double *area_a, double *frac_b, double *area_b);
+cr_status cr_regridder_from_esmf(cr_regridder **R, const char *path, void *stream);
 
 /* ---------------------------------------------- spectral elements -- */
 /* The ClimaCore extension's regridders (ext/ConservativeRegriddingClimaCoreExt.jl)

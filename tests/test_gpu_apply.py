@@ -365,3 +365,44 @@ def test_graph_captured_apply_replays_match_eager(cr, c2):
         assert torch.equal(y, ref32), rep
         assert torch.equal(y64, ref64), rep
         _ = (x * 2.0).sum().item()  # eager work between replays
+
+
+def test_capture_needs_prepared_set_up(cr):
+    """A cr_regrid whose direction has no plan yet fails inside a capture with
+    CR_UNSUPPORTED (its set-up allocates and synchronises); after
+    cr_regrid_prepare the same call captures, and every replay equals the
+    eager result."""
+    _case("capture_prepare")
+    L = cr._lib
+    src, dst = cr.Grid.from_points(cr.lonlat_points(90, 45)), cr.Grid.from_points(cr.lonlat_points(72, 36))
+    R = cr.Regridder(dst, src, radius=1.0)
+    n_dst, n_src = R.shape
+    k = 8
+    x = torch.tensor(_fields(n_src, k, 3).T.copy(), dtype=torch.float32, device=DEV)  # (k, n): cells fastest
+    y = torch.zeros(k, n_dst, dtype=torch.float32, device=DEV)
+    h = R._core.h
+
+    def call():
+        return L.lib.cr_regrid(h, 0, L.CR_F32, k, x.data_ptr(), 1, n_src, y.data_ptr(), 1, n_dst, 1,
+                               torch.cuda.current_stream().cuda_stream)
+
+    status = []
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        status.append(call())
+    assert status[0] == L.CR_UNSUPPORTED and "cr_regrid_prepare" in L.last_error()
+    assert L.lib.cr_regrid_prepare(h, 0, L.CR_F32, k, n_src, n_dst, None) == L.CR_OK
+    assert call() == L.CR_OK
+    torch.cuda.synchronize()
+    ref = y.clone()
+    g2 = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g2):
+        status.append(call())
+    assert status[1] == L.CR_OK
+    for rep in range(3):
+        y.zero_()
+        torch.cuda.synchronize()
+        g2.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(y, ref), rep
+        _ = (x * 2.0).sum().item()  # eager work between replays

@@ -161,3 +161,21 @@ def test_coarse_cells_with_thousands_of_candidates(cr, orc, coarse_dst):
     np.testing.assert_allclose(np.asarray(A.sum(0)).ravel(), R.src_areas, rtol=1e-12)
     if coarse_dst:
         assert np.diff(A.indptr).min() > 256 and np.diff(A.indptr).max() > 1024
+
+
+def test_zero_area_cells_are_counted(cr, orc):
+    """A destination cell collapsed to one point intersects nothing: its
+    candidate pairs are dropped and counted, the rest of A equals the oracle's."""
+    _case("zero_area_cell", src="3deg", dst="2deg + one collapsed cell")
+    dcells = cr.points_to_cells(cr.lonlat_points(180, 90))
+    ghost = dcells[5000, :1].repeat(4, axis=0)  # every vertex at one corner of cell 5000
+    dcells = np.concatenate([dcells, ghost[None]])
+    sp_ = cr.lonlat_points(120, 60)
+    R = cr.Regridder(cr.Grid.from_cells(dcells, convex=True), cr.Grid.from_points(sp_), radius=1.0)
+    st = R.stats()
+    assert st["n_zero_area_cells"] == 1 and st["n_zero_cell_pairs"] >= 1
+    assert st["n_nonfinite_pairs"] == 0 and st["n_nonfinite_cells"] == 0
+    rp, _, _ = R.csr_arrays()
+    assert rp[-1] == rp[-2]  # the collapsed cell's row is empty
+    ref = orc.build(dcells, cr.points_to_cells(sp_))
+    pattern_and_values(R, ref)
