@@ -166,7 +166,7 @@ def test_coarse_cells_with_thousands_of_candidates(cr, orc, coarse_dst):
 def test_zero_area_cells_are_counted(cr, orc):
     """A destination cell collapsed to one point intersects nothing: its
     candidate pairs are dropped and counted, the rest of A equals the oracle's."""
-    _case("zero_area_cell", src="3deg", dst="2deg + one collapsed cell")
+    _print_case("zero_area_cell", src="3deg", dst="2deg + one collapsed cell")
     dcells = cr.points_to_cells(cr.lonlat_points(180, 90))
     ghost = dcells[5000, :1].repeat(4, axis=0)  # every vertex at one corner of cell 5000
     dcells = np.concatenate([dcells, ghost[None]])
