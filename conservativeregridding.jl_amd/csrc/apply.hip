@@ -376,6 +376,29 @@ void apply(const cr_regridder *R, bool transpose, int dtype, int64_t nf, const v
     CR_HIP(hipGetLastError());
 }
 
+// True when the apply above, called with these arguments, would make no
+// per-direction set-up (nothing to allocate, nothing to synchronise).
+bool apply_ready(const cr_regridder *R, bool transpose, int dtype, int64_t nf, int64_t sf, int64_t df) {
+    const Csr &A = transpose ? R->bwd : R->fwd;
+    if (A.nrows == 0 || nf == 0) return true;
+    auto long_rows_ready = [&] {
+        std::lock_guard<std::mutex> lock(A.plan_mu);
+        return A.nlong >= 0;
+    };
+    if (dtype == CR_F64) {
+        const int v = A.f64_variant >= 0 ? A.f64_variant : kDefaultF64Variant;
+        return v == 1 ? plan_ready(A, kEven, kGeomS, false) : long_rows_ready();
+    }
+    if (sf == 1 && df == 1 && nf >= 64) return true;
+    int v = A.tuned_variant;
+    if (v < 0 || v >= kNumVariants) {
+        v = A.default_variant.load();
+        if (v < 0) return false;
+    }
+    const F32Variant &V = kF32Variants[v];
+    return V.tiled ? plan_ready(A, kRounds, V.geom, V.blocks != 0) : long_rows_ready();
+}
+
 // The per-direction set-up the apply above makes on its first call with these
 // arguments (default variant, tile plan or long-row list), without launching
 // the apply: afterwards such calls allocate nothing and never synchronise, so

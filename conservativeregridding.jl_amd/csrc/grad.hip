@@ -12,6 +12,9 @@
 // per-vertex gradients with FP64 atomics (order-nondeterministic in the last
 // bits).  Gradients are tangential: the areas depend on the vertices through
 // v/|v|.  Areas here are on the unit sphere; `area_scale` converts.
+#include <mutex>
+#include <string>
+
 #include "regridder.hpp"
 #include "sphgeom.hpp"
 
@@ -221,6 +224,114 @@ __global__ void k_div_rows(const double *__restrict__ gy, int64_t gc, int64_t gf
 void apply(const cr_regridder *R, bool transpose, int dtype, int64_t nf, const void *src, int64_t sc,
            int64_t sf, void *dst, int64_t dc, int64_t df, bool normalize, hipStream_t s,
            int force_variant);
+bool apply_ready(const cr_regridder *R, bool transpose, int dtype, int64_t nf, int64_t sf, int64_t df);
+
+// Scratch of the reverse mode: w (n_to x nfields doubles, the field adjoint's
+// input), coef (a double per non-zero) and rowid (an int32 per non-zero).
+struct BackwardScratch {
+    double *w = nullptr, *coef = nullptr;
+    int32_t *rowid = nullptr;
+};
+
+// The reverse mode on the given stream.  ws == nullptr: the scratch is
+// allocated for the call (stream-ordered) and the stream synchronised at the
+// end; otherwise the caller's scratch is used, nothing is allocated, every
+// per-direction plan must exist already, and the call returns without
+// synchronising.
+static void backward(const cr_regridder *R, const cr_grid *dst, const cr_grid *src, int32_t transpose,
+                     int32_t normalize, int64_t nfields, const double *x, int64_t xc, int64_t xf, const double *y,
+                     int64_t yc, int64_t yf, const double *gy, int64_t gc, int64_t gf, double *gx, int64_t gxc,
+                     int64_t gxf, double *g_dst_verts, double *g_src_verts, hipStream_t s,
+                     const BackwardScratch *ws) {
+    CR_REQUIRE(R && nfields > 0 && x && gy, CR_INVALID_ARGUMENT, "cr_regrid_backward: bad arguments");
+    CR_REQUIRE(xc > 0 && gc > 0 && (nfields == 1 || (xf > 0 && gf > 0)), CR_INVALID_ARGUMENT,
+               "cr_regrid_backward: bad strides");
+    const bool want_verts = g_dst_verts || g_src_verts;
+    const Csr &F = R->fwd;
+    const int64_t n_to = transpose ? F.ncols : F.nrows;
+    const double *a_to = transpose ? R->a_src.p : R->a_dst.p;
+    const int B = 256;
+    // field gradient: gx = A_dir^T (gy ./ a_to)
+    if (gx) {
+        CR_REQUIRE(gxc > 0 && (nfields == 1 || gxf > 0), CR_INVALID_ARGUMENT, "bad gx strides");
+        DevBuf<double> wb;
+        double *w = ws ? ws->w : nullptr;
+        if (!ws) {
+            wb.alloc_async((size_t)n_to * nfields, s);
+            w = wb.p;
+        } else {
+            CR_REQUIRE(apply_ready(R, !transpose, CR_F64, nfields, n_to, gxf), CR_UNSUPPORTED,
+                       "the field adjoint's plan is not made: call cr_regrid_prepare(R, !transpose, CR_F64, "
+                       "nfields, n_to, field stride of gx) first");
+        }
+        hipLaunchKernelGGL(k_div_rows, dim3(grid_for(n_to * nfields, B)), dim3(B), 0, s, gy, gc, gf, n_to,
+                           nfields, normalize ? a_to : nullptr, w);
+        CR_HIP(hipGetLastError());
+        apply(R, !transpose, CR_F64, nfields, w, 1, n_to, gx, gxc, gxf, false, s, -1);
+    }
+    if (!want_verts) {
+        if (!ws) CR_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    CR_REQUIRE(R->area_scale > 0.0, CR_UNSUPPORTED,
+               "vertex gradients need a regridder built from grids (cr_regridder_build)");
+    CR_REQUIRE(dst && src && dst->ncells == F.nrows && src->ncells == F.ncols, CR_DIMENSION_MISMATCH,
+               "cr_regrid_backward: grids do not match the regridder");
+    const int clip_nv = R->stats.clip_with_dst ? dst->nv : src->nv;
+    CR_REQUIRE(clip_nv <= 4, CR_UNSUPPORTED, "vertex gradients need clip cells with at most 4 vertices");
+    if (normalize) CR_REQUIRE(y, CR_INVALID_ARGUMENT, "cr_regrid_backward: the forward output y is needed");
+    const double kappa = R->area_scale;
+    DevBuf<double> coef_b;
+    DevBuf<int32_t> rowid_b;
+    double *coef = ws ? ws->coef : nullptr;
+    int32_t *rowid = ws ? ws->rowid : nullptr;
+    if (!ws) {
+        coef_b.alloc_async(F.nnz, s);
+        rowid_b.alloc_async(F.nnz, s);
+        coef = coef_b.p;
+        rowid = rowid_b.p;
+    }
+    hipLaunchKernelGGL(k_pair_rows, dim3(grid_for(F.nrows, B)), dim3(B), 0, s, F.rowptr.p, F.nrows, rowid);
+    if (F.nnz)
+        hipLaunchKernelGGL(k_pair_coef, dim3(grid_for(F.nnz, B)), dim3(B), 0, s, rowid, F.col.p, F.nnz,
+                           (int)(transpose != 0), (int)(normalize != 0), kappa, a_to, nfields, x, xc, xf, gy, gc,
+                           gf, coef);
+    if (F.nnz)
+        hipLaunchKernelGGL(k_clip_grad, dim3(grid_for(F.nnz, 64)), dim3(64),
+                           (size_t)kGradRows * dst->nv * 3 * sizeof(double), s, rowid, F.col.p, coef, F.nnz,
+                           dst->cells.p, dst->nv, dst->vidx.p, src->cells.p, src->nv, src->vidx.p,
+                           R->stats.clip_with_dst, g_dst_verts, g_src_verts);
+    double *g_to = transpose ? g_src_verts : g_dst_verts;
+    const cr_grid *gto = transpose ? src : dst;
+    if (normalize && g_to)
+        hipLaunchKernelGGL(k_area_grad, dim3(grid_for(n_to, B)), dim3(B), 0, s, gto->cells.p, gto->nv, gto->vidx.p,
+                           n_to, kappa, a_to, nfields, y, yc, yf, gy, gc, gf, g_to, gto->private_vertices);
+    CR_HIP(hipGetLastError());
+    if (!ws) CR_HIP(hipStreamSynchronize(s));
+}
+
+// The allocation-free reverse mode's scratch: one static array per device
+// (every GPU the library runs on keeps its copy until the process ends).  Host
+// code must use the per-device address from hipGetSymbolAddress, never the
+// array's name (that is a host-side stand-in, not a device address).
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static unsigned char *scratch_address() {
+    static std::mutex mu;
+    static unsigned char *ptr[256] = {};
+    int dev = 0;
+    CR_HIP(hipGetDevice(&dev));
+    CR_REQUIRE(dev >= 0 && dev < 256, CR_UNSUPPORTED, "device index out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        CR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        CR_REQUIRE(p, CR_INTERNAL, "hipGetSymbolAddress gave no address for the scratch array");
+        ptr[dev] = (unsigned char *)p;
+    }
+    return ptr[dev];
+}
 
 }  // namespace cr
 
@@ -228,60 +339,42 @@ using namespace cr;
 
 extern "C" {
 
+cr_status cr_regrid_backward_noalloc(const cr_regridder *R, const cr_grid *dst, const cr_grid *src,
+                                     int32_t transpose, int32_t normalize, int64_t nfields, const double *x,
+                                     int64_t xc, int64_t xf, const double *y, int64_t yc, int64_t yf,
+                                     const double *gy, int64_t gc, int64_t gf, double *gx, int64_t gxc,
+                                     int64_t gxf, double *g_dst_verts, double *g_src_verts, void *stream) {
+    return guard([&] {
+        CR_REQUIRE(R && nfields > 0, CR_INVALID_ARGUMENT, "cr_regrid_backward_noalloc: bad arguments");
+        const Csr &F = R->fwd;
+        const int64_t n_to = transpose ? F.ncols : F.nrows;
+        auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+        const size_t wbytes = gx ? up((size_t)n_to * (size_t)nfields * sizeof(double)) : 0;
+        const bool verts = g_dst_verts || g_src_verts;
+        const size_t cbytes = verts ? up((size_t)F.nnz * sizeof(double)) : 0;
+        const size_t rbytes = verts ? up((size_t)F.nnz * sizeof(int32_t)) : 0;
+        CR_REQUIRE(wbytes + cbytes + rbytes <= kNoAllocScratchBytes, CR_INVALID_ARGUMENT,
+                   "cr_regrid_backward_noalloc: the call needs " + std::to_string(wbytes + cbytes + rbytes) +
+                       " bytes of scratch, over its maximum of " + std::to_string(kNoAllocScratchBytes) +
+                       " (use cr_regrid_backward)");
+        unsigned char *base = scratch_address();
+        BackwardScratch ws;
+        ws.w = wbytes ? (double *)base : nullptr;
+        ws.coef = cbytes ? (double *)(base + wbytes) : nullptr;
+        ws.rowid = rbytes ? (int32_t *)(base + wbytes + cbytes) : nullptr;
+        backward(R, dst, src, transpose, normalize, nfields, x, xc, xf, y, yc, yf, gy, gc, gf, gx, gxc, gxf,
+                 g_dst_verts, g_src_verts, as_stream(stream), &ws);
+    });
+}
+
 cr_status cr_regrid_backward(const cr_regridder *R, const cr_grid *dst, const cr_grid *src,
                              int32_t transpose, int32_t normalize, int64_t nfields, const double *x,
                              int64_t xc, int64_t xf, const double *y, int64_t yc, int64_t yf,
                              const double *gy, int64_t gc, int64_t gf, double *gx, int64_t gxc,
                              int64_t gxf, double *g_dst_verts, double *g_src_verts, void *stream) {
     return guard([&] {
-        CR_REQUIRE(R && nfields > 0 && x && gy, CR_INVALID_ARGUMENT, "cr_regrid_backward: bad arguments");
-        CR_REQUIRE(xc > 0 && gc > 0 && (nfields == 1 || (xf > 0 && gf > 0)), CR_INVALID_ARGUMENT,
-                   "cr_regrid_backward: bad strides");
-        const bool want_verts = g_dst_verts || g_src_verts;
-        hipStream_t s = as_stream(stream);
-        const Csr &F = R->fwd;
-        const int64_t n_to = transpose ? F.ncols : F.nrows, n_from = transpose ? F.nrows : F.ncols;
-        const double *a_to = transpose ? R->a_src.p : R->a_dst.p;
-        const int B = 256;
-        // field gradient: gx = A_dir^T (gy ./ a_to)
-        if (gx) {
-            CR_REQUIRE(gxc > 0 && (nfields == 1 || gxf > 0), CR_INVALID_ARGUMENT, "bad gx strides");
-            DevBuf<double> w((size_t)n_to * nfields, s);
-            hipLaunchKernelGGL(k_div_rows, dim3(grid_for(n_to * nfields, B)), dim3(B), 0, s, gy, gc, gf, n_to,
-                               nfields, normalize ? a_to : nullptr, w.p);
-            CR_HIP(hipGetLastError());
-            apply(R, !transpose, CR_F64, nfields, w.p, 1, n_to, gx, gxc, gxf, false, s, -1);
-            (void)n_from;
-        }
-        if (!want_verts) return;
-        CR_REQUIRE(R->area_scale > 0.0, CR_UNSUPPORTED,
-                   "vertex gradients need a regridder built from grids (cr_regridder_build)");
-        CR_REQUIRE(dst && src && dst->ncells == F.nrows && src->ncells == F.ncols, CR_DIMENSION_MISMATCH,
-                   "cr_regrid_backward: grids do not match the regridder");
-        const int clip_nv = R->stats.clip_with_dst ? dst->nv : src->nv;
-        CR_REQUIRE(clip_nv <= 4, CR_UNSUPPORTED, "vertex gradients need clip cells with at most 4 vertices");
-        if (normalize) CR_REQUIRE(y, CR_INVALID_ARGUMENT, "cr_regrid_backward: the forward output y is needed");
-        const double kappa = R->area_scale;
-        DevBuf<double> coef(F.nnz, s);
-        DevBuf<int32_t> rowid(F.nnz, s);
-        hipLaunchKernelGGL(k_pair_rows, dim3(grid_for(F.nrows, B)), dim3(B), 0, s, F.rowptr.p, F.nrows, rowid.p);
-        if (F.nnz)
-            hipLaunchKernelGGL(k_pair_coef, dim3(grid_for(F.nnz, B)), dim3(B), 0, s, rowid.p, F.col.p, F.nnz,
-                               (int)(transpose != 0), (int)(normalize != 0), kappa, a_to, nfields, x, xc, xf, gy,
-                               gc, gf, coef.p);
-        if (F.nnz)
-            hipLaunchKernelGGL(k_clip_grad, dim3(grid_for(F.nnz, 64)), dim3(64),
-                               (size_t)kGradRows * dst->nv * 3 * sizeof(double), s, rowid.p, F.col.p, coef.p,
-                               F.nnz, dst->cells.p, dst->nv, dst->vidx.p, src->cells.p, src->nv, src->vidx.p,
-                               R->stats.clip_with_dst, g_dst_verts, g_src_verts);
-        double *g_to = transpose ? g_src_verts : g_dst_verts;
-        const cr_grid *gto = transpose ? src : dst;
-        if (normalize && g_to)
-            hipLaunchKernelGGL(k_area_grad, dim3(grid_for(n_to, B)), dim3(B), 0, s, gto->cells.p, gto->nv,
-                               gto->vidx.p, n_to, kappa, a_to, nfields, y, yc, yf, gy, gc, gf, g_to,
-                               gto->private_vertices);
-        CR_HIP(hipGetLastError());
-        CR_HIP(hipStreamSynchronize(s));
+        backward(R, dst, src, transpose, normalize, nfields, x, xc, xf, y, yc, yf, gy, gc, gf, gx, gxc, gxf,
+                 g_dst_verts, g_src_verts, as_stream(stream), nullptr);
     });
 }
 

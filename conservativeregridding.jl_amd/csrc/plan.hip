@@ -754,6 +754,11 @@ const TilePlan &tile_plan(const Csr &A, int style, int geom, hipStream_t s, bool
     return *slot;
 }
 
+bool plan_ready(const Csr &A, int style, int geom, bool blocks) {
+    std::lock_guard<std::mutex> lock(A.plan_mu);
+    return (bool)A.tiles[style][geom][blocks ? 1 : 0];
+}
+
 void drop_tile_plan(const Csr &A) {
     std::lock_guard<std::mutex> lock(A.plan_mu);
     for (auto &byStyle : A.tiles)

@@ -64,6 +64,8 @@ struct Csr {
 const TilePlan &tile_plan(const Csr &A, int style, int geom, hipStream_t s, bool blocks = false);
 // Forget the plan (after A's values change).
 void drop_tile_plan(const Csr &A);
+// Whether that plan is made already (making it allocates and synchronises).
+bool plan_ready(const Csr &A, int style, int geom, bool blocks);
 
 }  // namespace cr
 

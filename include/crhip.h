@@ -288,6 +288,25 @@ cr_status cr_regrid_backward(const cr_regridder *R, const cr_grid *dst, const cr
                              int64_t gxf, double *g_dst_verts, double *g_src_verts, void *stream);
 
 /*
+ * cr_regrid_backward without allocating (no reference counterpart: for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results; its
+ * scratch (n_to * nfields doubles for gx, 12 bytes per non-zero for vertex
+ * gradients, each rounded up to 256 bytes; n_to = the output cell count of the
+ * direction) lives in one static device array of 64 MiB per GPU, kept until
+ * the process ends.  Larger requests fail with CR_INVALID_ARGUMENT.  When gx is
+ * wanted, the field adjoint's plan must exist: call
+ * cr_regrid_prepare(R, !transpose, CR_F64, nfields, n_to, gx's field stride)
+ * first, else CR_UNSUPPORTED.  Does not synchronise the stream.  Calls on one
+ * GPU share the array: calls that overlap (other streams, other threads) race.
+ */
+cr_status cr_regrid_backward_noalloc(const cr_regridder *R, const cr_grid *dst, const cr_grid *src,
+                                     int32_t transpose, int32_t normalize, int64_t nfields, const double *x,
+                                     int64_t xc, int64_t xf, const double *y, int64_t yc, int64_t yf,
+                                     const double *gy, int64_t gc, int64_t gf, double *gx, int64_t gxc,
+                                     int64_t gxf, double *g_dst_verts, double *g_src_verts, void *stream);
+
+/*
  * Autotuner for the FP32 cells-fastest batched apply (no reference
  * counterpart: the reference's slice loop has no tunable).  Times every kernel
  * variant on the given device fields (dst is overwritten with the regridded
