@@ -128,10 +128,12 @@ struct VertexSink {
 // pair; other contributions use global FP64 atomics directly.
 constexpr int kGradRows = 16;
 
-// Two waves per SIMD: the compiler then keeps 256 registers and spills ~70 to
-// scratch (the kernel's 300 registers allow one wave), and the second wave
-// hides the atomics' latency (9.9 against 12.1 ms on the benchmark; three or
-// four waves spill more and run slower).
+// Two waves per SIMD (256 registers): the second wave hides the atomics' and
+// the FP64 chain's latency.  The clip pipeline's edge normals live in LDS (6 KB
+// per workgroup), which cuts the register spills from 73 to 24; moving its
+// first vertices and clip vertices there too removes the rest but lowers the
+// occupancy LDS allows and runs slower (C3 vertex pass: 8.4 ms with the
+// normals in registers, 7.1 ms as here, 9.1 ms with none spilled).
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_clip_grad(
     const int32_t *__restrict__ rowid, const int32_t *__restrict__ col, const double *__restrict__ coef,
     int64_t nnz, const double *__restrict__ dst_cells, int dst_nv, const int32_t *__restrict__ dst_vi,
@@ -139,6 +141,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
     int clip_is_dst, double *__restrict__ g_dst, double *__restrict__ g_src) {
     extern __shared__ double lacc[];  // kGradRows * dst_nv * 3
     __shared__ double own[64 * 12];
+    __shared__ d3 nrm_lds[64 * 4];  // each thread's 4 clip-edge normals
     const int64_t k0 = (int64_t)blockIdx.x * 64, k = k0 + threadIdx.x;
     const int64_t r0 = rowid[k0];
     const int nacc = kGradRows * dst_nv * 3;
@@ -162,7 +165,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
             sink = VertexSink{dp, sp, dst_nv, src_nv, dst_vi + (int64_t)dst_nv * d, src_vi + (int64_t)src_nv * s,
                               g_dst, g_src, mine, nullptr, g_src ? my_own : nullptr, nullptr};
         }
-        clip_area_backward<4, VertexSink>(sink.clip, sink.nc, sink.subj, sink.ns, c, sink);
+        clip_area_backward<4, VertexSink>(sink.clip, sink.nc, sink.subj, sink.ns, c, sink,
+                                          nrm_lds + 4 * threadIdx.x);
         // the per-pair sums of the source cell's 4 slots, one atomic per component
         if (sink.own_clip || sink.own_subj) {
             const int32_t *vi = sink.own_clip ? sink.clip_vi : sink.subj_vi;

@@ -333,7 +333,8 @@ CR_HD d3 tangential(const d3 &g, const d3 &x) { return g - dot(g, x) * x; }
 
 template <int NC, class Sink>
 struct GradPipe {
-    d3 anchor[NC], nrm[NC];
+    d3 anchor[NC];
+    d3 *nrm;  // the NC edge normals, in caller memory (LDS) to spare registers
     d3 first[NC], prev[NC];
     double sfirst[NC], sprev[NC];
     int32_t lfirst[NC], lprev[NC];
@@ -419,11 +420,13 @@ CR_HD void gpipe_close(GradPipe<NC, Sink> &P) {
     }
 }
 
-// Scatter coef * d(area)/d(vertex) of (subject ∩ clip) through `sink`.
+// Scatter coef * d(area)/d(vertex) of (subject ∩ clip) through `sink`;
+// nrm_store holds NC normals (the caller's, e.g. a per-thread slice of LDS).
 template <int NC, class Sink>
 CR_HD void clip_area_backward(const double *clip, int nc, const double *subj, int ns, double coef,
-                              Sink &sink) {
+                              Sink &sink, d3 *nrm_store) {
     GradPipe<NC, Sink> P;
+    P.nrm = nrm_store;
     P.ns = ns;
     P.coef = coef;
     P.sink = &sink;
