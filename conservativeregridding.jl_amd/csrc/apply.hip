@@ -300,10 +300,12 @@ constexpr F32Variant kF32Variants[] = {
     {true, 4, false, 0, 0, 1},   {true, 4, false, 4, 0, 1},   {true, 4, 2, 0, 0, 1},
     {true, 4, 2, 4, 0, 1},       {true, 4, 2, 0, 0, 1, 1},    {true, 4, 2, 4, 0, 1, 1},
     {true, 4, 2, 4, 1, 1},       {true, 4, 1, 4, 1, 1},       {true, 4, 2, 2, 1, 1},
-    {true, 4, 2, 4, 1, 1, 1}};
+    {true, 4, 2, 4, 1, 1, 1},    {true, 4, 1, 0, 0, 1, 1}};
 constexpr int kNumVariants = sizeof(kF32Variants) / sizeof(kF32Variants[0]);
-constexpr int kDefaultF32Variant = 21;  // 1024-row tiles, 4 fields per stage, FP64 sums of FP32 A
-constexpr int kBlockF32Variant = 23;    // the same on the block-ordered plan
+// 1024-row tiles, 4 fields per stage, products and sums in FP64 from the FP64 A,
+// field chunks by default_chunks
+constexpr int kDefaultF32Variant = 15;
+constexpr int kBlockF32Variant = 28;  // the same on the block-ordered plan
 
 // The default FP32 variant of a matrix, chosen once: the block-ordered plan
 // when the plain plan's tiles fetch more than 3 source lines (32 columns each)
@@ -542,6 +544,8 @@ cr_status cr_regrid_tune(cr_regridder *R, int32_t transpose, int64_t nfields, co
         int best = -1;
         float best_t = 1e30f;
         for (int v = 0; v < kNumVariants; ++v) {
+            // the variants with the default's numerics: products and sums in FP64 from the FP64 A
+            if (!(kF32Variants[v].tiled && kF32Variants[v].acc == 1)) continue;
             // one warm-up launch, then the median of three timed launches
             apply(R, transpose != 0, CR_F32, nfields, src, src_cell_stride, src_field_stride, dst,
                   dst_cell_stride, dst_field_stride, true, s, v);

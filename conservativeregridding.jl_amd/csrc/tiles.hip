@@ -316,12 +316,18 @@ static void launch_f(const TilePlan &P, int nchunks, int chunk_major, const T *x
                            x, sc, sf, y, dc, df, nf, scale, P.rowmap.p);
 }
 
-// Workgroups for a tile launch: enough (tile, chunk) pairs to keep every CU
-// busy several times over when the plan has few tiles.
-int default_chunks(const TilePlan &P, int64_t nf, int F) {
-    const int64_t want = (P.geom == kGeomL ? 4 : 8) * 256;  // 32 waves per CU
+// Field chunks of a tile launch (each chunk is a workgroup per tile): enough
+// (tile, chunk) workgroups to keep every CU busy several times over when the
+// plan has few tiles, but each chunk past the first reads the tile's slots
+// again, so no more chunks than keep that re-read within a tenth of the
+// launch's field bytes (a plan with many tiles over a few dozen fields, whose
+// slots are a large share of the traffic, gets one chunk).
+int default_chunks(const TilePlan &P, int64_t nf, int F, int slot_bytes, int64_t field_bytes) {
+    const int64_t want = P.geom == kGeomL ? 8192 : 2048;
     int64_t c = (want + P.ntiles - 1) / std::max<int64_t>(1, P.ntiles);
     c = std::max<int64_t>(1, std::min<int64_t>(c, (nf + F - 1) / F));
+    const double reread = (double)P.nslots * slot_bytes;
+    while (c > 1 && (double)(c - 1) * reread > 0.1 * (double)field_bytes) --c;
     return (int)c;
 }
 
@@ -336,7 +342,10 @@ void launch_tiles(const TilePlan &P, int F, int acc, int nchunks, int chunk_majo
                   int64_t ncols, hipStream_t s) {
     if (P.ntiles == 0) return;
     const bool acc64 = acc != 0;
-    if (nchunks <= 0) nchunks = default_chunks(P, nf, F);
+    if (nchunks <= 0) {
+        const int slot_bytes = 4 + (sizeof(T) == 4 && acc != 1 ? 4 : 8);  // key + value as the kernel reads it
+        nchunks = default_chunks(P, nf, F, slot_bytes, nf * (nrows + ncols) * (int64_t)sizeof(T));
+    }
     nchunks = (int)std::max<int64_t>(1, std::min<int64_t>(nchunks, (nf + F - 1) / F));
 #define CR_TILE_ARGS P, nchunks, chunk_major, x, sc, sf, y, dc, df, nf, scale, nrows, ncols, s
     if constexpr (sizeof(T) == 4) {

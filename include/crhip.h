@@ -309,9 +309,12 @@ cr_status cr_regrid_backward_noalloc(const cr_regridder *R, const cr_grid *dst, 
 /*
  * Autotuner for the FP32 cells-fastest batched apply (no reference
  * counterpart: the reference's slice loop has no tunable).  Times every kernel
- * variant on the given device fields (dst is overwritten with the regridded
- * result) and stores the fastest in the handle for that direction; later
- * cr_regrid calls of that direction use it.  Synchronises the stream.
+ * variant with the default's numerics (products and sums in FP64 from the FP64
+ * A: variants 8-17, 25 and 28) on the given device fields (dst is overwritten
+ * with the regridded result) and stores the fastest in the handle for that
+ * direction; later cr_regrid calls of that direction use it.  Variants that
+ * round A to FP32 or sum in FP32 are used only when pinned
+ * (cr_regrid_set_variant).  Synchronises the stream.
  */
 cr_status cr_regrid_tune(cr_regridder *R, int32_t transpose, int64_t nfields, const void *src,
                          int64_t src_cell_stride, int64_t src_field_stride, void *dst,
@@ -326,18 +329,24 @@ cr_status cr_regrid_tune(cr_regridder *R, int32_t transpose, int64_t nfields, co
  *           tile kernel with 2, 4, 8 fields per LDS stage and FP32 Kahan sums;
  *           8-10 the tile kernel with 2, 4, 8 fields per stage and FP64 sums;
  *           11-14 variant 9 with 4 field chunks, 8 and 16 chunks
- *           swept one at a time, and a single chunk; 15-17 FP64 sums on
- *           1024-row tiles (512 threads) with 4, 2 and 4 fields per stage (17:
- *           4 field chunks); 18-19 FP32 Kahan sums on 1024-row tiles with 4
- *           fields per stage (19: 4 field chunks); 20-21 FP64 sums of A
+ *           swept one at a time, and a single chunk; 15-17 FP64 sums of the
+ *           FP64 A on 1024-row tiles (512 threads) with 4, 2 and 4 fields per
+ *           stage (17: 4 field chunks); 18-19 FP32 Kahan sums on 1024-row tiles
+ *           with 4 fields per stage (19: 4 field chunks); 20-21 FP64 sums of A
  *           rounded to FP32 on 1024-row tiles, 4 fields per stage (21: 4 field
  *           chunks); 22-23 variants 20-21 on a plan whose rows are 32-row
  *           blocks sorted by their smallest column (for rows that are strips
- *           of a row-major grid over a source in another order).  The default
- *           is 21, or 23 when 21's tiles fetch over 3 source lines per 32
- *           distinct source cells and 23's fetch fewer (decided once per
- *           direction at its first default apply).  FP32 fields with unit field strides and >= 64
- *           fields always use the fields-fastest kernel.
+ *           of a row-major grid over a source in another order); 24-26
+ *           variants 21, 17 and 21 with 4, 4 and 2 field chunks swept one at a
+ *           time; 27 variant 23 swept likewise; 28 variant 15 on the
+ *           block-ordered plan.  Variants without a fixed chunk count take as
+ *           many chunks as keep 8192 workgroups busy, fewer where re-reading a
+ *           tile's slots per chunk would pass a tenth of the launch's field
+ *           bytes.  The default is 15 (A used in FP64), or 28 when 15's tiles
+ *           fetch over 3 source lines per 32 distinct source cells and 28's
+ *           fetch fewer (decided once per direction at its first default
+ *           apply).  FP32 fields with unit field strides and >= 64 fields
+ *           always use the fields-fastest kernel.
  *   CR_F64: 0 row kernel, 1 staged tile kernel (default).
  */
 int32_t cr_regrid_variant_count(int32_t dtype);

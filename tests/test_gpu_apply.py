@@ -206,6 +206,7 @@ def test_autotuned_variant_is_accurate(cr, c2):
     y = torch.zeros(k, n_dst, dtype=torch.float32, device=DEV).T
     variant, ms = cr.regrid_tune(y, R, x)
     assert variant >= 0 and ms > 0
+    assert variant in (8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 25, 28)  # FP64 products and sums of the FP64 A
     y.zero_()
     cr.regrid_(y, R, x)
     ref = (A @ x.cpu().numpy().astype(np.float64)) / R.dst_areas[:, None]
@@ -289,7 +290,7 @@ def test_default_variant_on_row_strips_over_nested_sources(cr):
         ref = torch.zeros(k, n_out, dtype=torch.float64, device=DEV)
         cr.regrid_(ref.T, RR, x64.T)
         x = x64.float()
-        for v in (-1, 22, 23):
+        for v in (-1, 22, 23, 28):
             cr.regrid_set_variant(RR, v)
             y = torch.full((k, n_out), float("nan"), dtype=torch.float32, device=DEV)
             cr.regrid_(y.T, RR, x.T)

@@ -4,7 +4,9 @@ then
 
   * the batched FP32 apply of 256 seeded harmonic fields laid out [n, 64, 4]
     (cells fastest, dims=1), forward and through transpose(R), with the
-    default kernel and the pinned FP64-A (17) and FP32-A (21) tile variants,
+    default kernel (variant 15: products and sums in FP64 from the FP64 A) and
+    the pinned variants 17 (the same, 4 field chunks), 21 (A rounded to FP32),
+    23 and 28 (21 and 15 on the block-ordered plan),
     against the oracle's FP64 SpMV on the oracle-built CSR for 8 sampled
     fields: relative field error <= 1e-6, conservation drift <= 1e-7;
   * one FP64 field forward and transposed against the oracle to 1e-12;
@@ -52,7 +54,7 @@ def _fp32_report(got, x64, ref_csr, a_to, a_from):
     return worst_err, worst_drift
 
 
-@pytest.mark.parametrize("variant", [-1, 17, 21])
+@pytest.mark.parametrize("variant", [-1, 17, 21, 23, 28])
 def test_c3_batched_fp32_forward_and_transpose(cr, c3, variant):
     _case("c3_batched_fp32", variant=variant, fields=256, layout="[n,64,4]")
     R, ref, refT, X2 = c3
