@@ -12,6 +12,8 @@ from .grids import (Grid, NESTED, RING, as_grid, cubed_sphere_points, healpix_ce
                     polygons_to_cells, polygons_width, ring_full_polygons, ring_latitudes, ring_longitudes)
 from .esmf import read_esmf_weights, regridder_from_esmf, save_esmf_weights, write_esmf_arrays
 from .se import SpectralElementSpace, se_moments
+from .fields import (ExampleFieldFunction, GulfStreamField, HarmonicField, LongitudeField, SinusoidField, VortexField,
+                     cell_averages)
 from .regridder import (EARTH_RADIUS, Regridder, areas, normalize_, regrid, regrid_, regrid_plan,
                         regrid_tune, regrid_backward, regrid_variants, regrid_set_variant,
                         size, transpose)
@@ -20,6 +22,8 @@ __all__ = [
     "Regridder", "regrid_", "regrid", "regrid_plan", "regrid_tune", "regrid_backward", "regrid_variants",
     "regrid_set_variant", "transpose", "normalize_", "areas", "size", "save_esmf_weights", "write_esmf_arrays", "read_esmf_weights", "regridder_from_esmf",
     "SpectralElementSpace", "se_moments",
+    "ExampleFieldFunction", "LongitudeField", "SinusoidField", "HarmonicField", "GulfStreamField", "VortexField",
+    "cell_averages",
     "Grid", "NESTED", "RING", "as_grid", "lonlat_points", "cubed_sphere_points", "healpix_cells",
     "healpix_nest2ring", "healpix_ring2nest", "points_to_cells", "lonlat_to_unit", "polygons_to_cells",
     "polygons_width", "flatten_tree", "ring_full_polygons", "ring_latitudes", "ring_longitudes",

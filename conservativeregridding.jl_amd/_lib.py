@@ -109,6 +109,7 @@ SIGNATURES = {
                                    c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp]),
     "cr_regrid_backward_noalloc": (c_i32, [c_vp, c_vp, c_vp, c_i32, c_i32, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64,
                                            c_i64, c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp]),
+    "cr_cell_averages": (c_i32, [c_vp, c_i32, _dp, c_i32, c_i32, c_vp, c_vp]),
     "cr_regrid_prepare": (c_i32, [c_vp, c_i32, c_i32, c_i64, c_i64, c_i64, c_vp]),
     "cr_regrid_tune": (c_i32, [c_vp, c_i32, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, _i32p,
                                ctypes.POINTER(ctypes.c_float), c_vp]),

@@ -34,18 +34,13 @@
 
 #include "regridder.hpp"
 #include "sphgeom.hpp"
+#include "trirule.hpp"
 
 namespace cr {
 
 constexpr int kSeMaxNq = 8;
-constexpr int kSeMaxRule = 16;
 constexpr int kSeMaxCellV = 32;            // cells of at most this many vertices
 constexpr int kSeMaxPoly = kSeMaxCellV + 8;  // clipped polygon buffer
-
-struct TriRule {
-    int n;
-    double l1[kSeMaxRule], l2[kSeMaxRule], l3[kSeMaxRule], w[kSeMaxRule];
-};
 
 struct NodeSet {
     int n;
@@ -121,6 +116,8 @@ bool dunavant(int degree, TriRule &r) {
     }
 }
 
+}  // namespace
+
 // reference_rule: the rule of this degree, or of the next degree that exists.
 TriRule tri_rule(int degree) {
     TriRule r{};
@@ -129,6 +126,8 @@ TriRule tri_rule(int degree) {
     throw Error{CR_UNSUPPORTED, "triangle quadrature degree " + std::to_string(degree) +
                                     " not available (max is 8)"};
 }
+
+namespace {
 
 // Gauss-Lobatto-Legendre nodes (ascending) and weights: -1, the roots of
 // P'_{n-1} (Newton from the Chebyshev-Lobatto points), 1; w = 2 / (n(n-1) P_{n-1}^2).

@@ -373,7 +373,7 @@ class Grid:
         h, destroy = getattr(self, "_h", None), getattr(self, "_destroy", None)
         if h is not None and h.value and destroy is not None:
             destroy(h)
-            self._h = ctypes.c_void_p()
+            self._h = type(h)()  # no module globals: they may be gone at interpreter exit
 
     def __repr__(self):
         return f"Grid({self.kind}, ncells={self.ncells}, nv={self.nv}, convex={self.convex})"

@@ -84,7 +84,7 @@ class _Core:
         h, destroy = getattr(self, "h", None), getattr(self, "_destroy", None)
         if h is not None and h.value and destroy is not None:
             destroy(h)
-            self.h = ctypes.c_void_p()
+            self.h = type(h)()  # no module globals: they may be gone at interpreter exit
 
 
 def _raise_intersection_failure(dg: Grid, sg: Grid):
@@ -468,7 +468,8 @@ def regrid_plan(dst_field, R: Regridder, src_field, *, dims: int = 1, normalize:
 def regrid_tune(dst_field, R: Regridder, src_field, *, dims: int = 1):
     """Autotune the FP32 cells-fastest batched apply for this direction of ``R``
     on these device fields (``dst_field`` receives the regridded result).  The
-    fastest kernel variant is kept in the regridder for later calls.  Returns
+    fastest kernel variant with the default's numerics (FP64 products and sums
+    from the FP64 A) is kept in the regridder for later calls.  Returns
     (variant index, milliseconds per launch)."""
     import torch
     regrid_(dst_field, R, src_field, dims=dims)  # validation
@@ -493,11 +494,12 @@ def regrid_set_variant(R: Regridder, variant: int, dtype="float32") -> None:
     field dtype (``-1`` restores the default).  FP32: 0-4 row kernel with 1-16
     fields per thread, 5-7 staged tile kernel with 2/4/8 fields per stage and
     Kahan sums, 8-10 the same with FP64 sums, 11-14 variant 9 with other field
-    chunkings, 15-17 FP64 sums on 1024-row tiles, 18-19 Kahan sums on 1024-row
-    tiles, 20-21 FP64 sums of A rounded to FP32 on 1024-row tiles, 22-23 the
-    same on 32-row blocks sorted by their smallest column (default: 21, or 23
-    when its plan fetches fewer source lines than 21's and 21's fetch over 3
-    lines per 32 distinct source cells);
+    chunkings, 15-17 FP64 sums of the FP64 A on 1024-row tiles, 18-19 Kahan
+    sums on 1024-row tiles, 20-21 FP64 sums of A rounded to FP32 on 1024-row
+    tiles, 22-23 the same on 32-row blocks sorted by their smallest column,
+    24-27 field chunks swept one at a time, 28 variant 15 on the block order
+    (default: 15, or 28 when its plan fetches fewer source lines than 15's and
+    15's fetch over 3 lines per 32 distinct source cells);
     FP64: 0 row kernel, 1 tile kernel (default)."""
     L.check(L.lib.cr_regrid_set_variant(R._core.h, 1 if R._t else 0, _dtype_code(dtype), int(variant)),
             "regrid_set_variant")

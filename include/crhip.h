@@ -472,6 +472,30 @@ cr_status cr_se_regridder_from_fv(cr_regridder **R, const cr_grid *elements, con
                                   int32_t tri_degree, const double *node_weights, const cr_build_opts *opts,
                                   void *stream);
 
+/* ------------------------------------------- example-field cell averages -- */
+/* The reference's example fields (src/utils/example_data.jl), as functions of
+ * (lon, lat) in degrees. */
+typedef enum {
+    CR_FIELD_LONGITUDE = 0,   /* LongitudeField: lon */
+    CR_FIELD_SINUSOID = 1,    /* SinusoidField(; dp_length, coef, coefmult) */
+    CR_FIELD_HARMONIC = 2,    /* HarmonicField: 2 + sin(2 lat)^16 cos(16 lon) */
+    CR_FIELD_GULF_STREAM = 3, /* GulfStreamField(; dp_length, coef, gf_coef, gf_ori_lon, gf_ori_lat,
+                                 gf_end_lon, gf_end_lat, gf_dmp_lon, gf_dmp_lat) */
+    CR_FIELD_VORTEX = 4       /* VortexField(; lon0_rad, lat0_rad, r0, d, t) */
+} cr_example_field;
+
+/* Cell averages int f dOmega / int dOmega of an example field over every cell
+ * of a grid, into the DEVICE array out (ncells doubles): the test helpers'
+ * set_field_values! with SphericalPolygonIntegrator (fan triangles from the
+ * first vertex, x = p/|p| over each, the symmetric rule of `degree`, 8 by the
+ * reference's default;
+ * lib/ConservativeRegriddingTestHelpers/src/ConservativeRegriddingTestHelpers.jl:84-128).
+ * lon = atan2(y, x), lat = asin(z) in degrees.  params: NULL for the
+ * reference's defaults, else nparams values in the keyword order above (3, 9
+ * or 5; none for the longitude and harmonic fields).  Zero-area cells get 0. */
+cr_status cr_cell_averages(const cr_grid *grid, int32_t field, const double *params, int32_t nparams,
+                           int32_t degree, double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

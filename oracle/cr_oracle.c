@@ -49,6 +49,13 @@
 
 typedef struct { double x, y, z; } v3;
 
+/* FP64 operation counting for the roofline of the device clip: when g_fl is
+ * set (orc_set_flop_counting), the clip and area below add the add, sub, mul,
+ * div and sqrt operations they perform to *g_fl (atan2 counts as one). */
+static _Thread_local int64_t *g_fl = NULL;
+static int g_count_flops = 0;
+#define FL(n) do { if (g_fl) *g_fl += (n); } while (0)
+
 static inline v3 v3_sub(v3 a, v3 b) { v3 r = {a.x - b.x, a.y - b.y, a.z - b.z}; return r; }
 static inline v3 v3_add(v3 a, v3 b) { v3 r = {a.x + b.x, a.y + b.y, a.z + b.z}; return r; }
 static inline v3 v3_scale(v3 a, double s) { v3 r = {a.x * s, a.y * s, a.z * s}; return r; }
@@ -73,6 +80,7 @@ static double now_s(void) {
 
 /* Solid angle of the spherical triangle (a, b, c), signed (positive if CCW). */
 static double tri_excess(v3 a, v3 b, v3 c) {
+    FL(40); /* 2 subs, cross, dot; 3 dots + 3 adds; atan2, scale */
     double num = v3_dot(a, v3_cross(v3_sub(b, a), v3_sub(c, a)));
     double den = 1.0 + v3_dot(a, b) + v3_dot(b, c) + v3_dot(c, a);
     return 2.0 * atan2(num, den);
@@ -81,6 +89,7 @@ static double tri_excess(v3 a, v3 b, v3 c) {
 static double ring_area(const v3 *p, int n) {
     double s = 0.0;
     for (int i = 1; i + 1 < n; ++i) s += tri_excess(p[0], p[i], p[i + 1]);
+    FL(n > 2 ? n - 2 : 0);
     return s;
 }
 
@@ -105,10 +114,12 @@ static int sh_clip(const v3 *S, int ns, const v3 *C, int nc, v3 *out, v3 *tmp) {
     for (int k = 0; k < nc && m > 0; ++k) {
         v3 a = C[k], b = C[(k + 1) % nc];
         v3 n = v3_cross(a, v3_sub(b, a));
+        FL(18); /* normal (sub, cross), its norm (dot, sqrt) */
         if (sqrt(v3_dot(n, n)) < EDGE_EPS) continue; /* degenerate edge */
         int w = 0;
         v3 p = cur[m - 1];
         double sp = v3_dot(n, v3_sub(p, a));
+        FL(8 + 8 * m); /* one side test (sub, dot) per vertex, plus the first */
         for (int i = 0; i < m; ++i) {
             v3 q = cur[i];
             double sq = v3_dot(n, v3_sub(q, a));
@@ -116,11 +127,13 @@ static int sh_clip(const v3 *S, int ns, const v3 *C, int nc, v3 *out, v3 *tmp) {
                 if (sp < 0.0) {
                     double t = sp / (sp - sq);
                     nxt[w++] = v3_normalize(v3_add(p, v3_scale(v3_sub(q, p), t)));
+                    FL(21); /* t (sub, div), lerp (sub, scale, add), normalise (dot, sqrt, div, scale) */
                 }
                 nxt[w++] = q;
             } else if (sp >= 0.0) {
                 double t = sp / (sp - sq);
                 nxt[w++] = v3_normalize(v3_add(p, v3_scale(v3_sub(q, p), t)));
+                FL(21);
             }
             if (w >= MAXV - 2) return -1;
             p = q;
@@ -248,6 +261,8 @@ typedef struct {
     double *val;
     double *a_dst, *a_src;
     double t_total, t_areas, t_pairs;
+    int64_t clip_flops;      /* FP64 operations of every candidate's clip and area (when counted) */
+    int64_t clip_flops_kept; /* the same over the candidates kept as non-zeros */
     int32_t status;
 } orc_result;
 
@@ -268,6 +283,7 @@ typedef struct {
     int32_t *row_count;
     _Atomic int64_t next_row;
     _Atomic int64_t n_candidates;
+    _Atomic int64_t clip_flops, clip_flops_kept;
     _Atomic int32_t failed;
 } build_ctx_t;
 
@@ -286,7 +302,8 @@ static void *build_worker(void *arg) {
     entry_t *ebuf = malloc(sizeof(entry_t) * 1024);
     int64_t ebuf_cap = 1024;
     v3 S[MAXV], C[MAXV], out[MAXV], tmp[MAXV];
-    int64_t local_cand = 0;
+    int64_t local_cand = 0, local_fl = 0, local_fl_kept = 0;
+    g_fl = g_count_flops ? &local_fl : NULL;
     for (;;) {
         int64_t r0 = atomic_fetch_add(&c->next_row, ROW_CHUNK);
         if (r0 >= c->row_end) break;
@@ -329,6 +346,7 @@ static void *build_worker(void *arg) {
                 int nc = c->clip_is_dst ? c->dst_nv : c->src_nv;
                 for (int i = 0; i < ns; ++i) S[i] = v3_load(subj + 3 * i);
                 for (int i = 0; i < nc; ++i) C[i] = v3_load(clip + 3 * i);
+                const int64_t fl0 = local_fl;
                 int m = sh_clip(S, ns, C, nc, out, tmp);
                 if (m < 0) { atomic_store(&c->failed, 1); continue; }
                 double a = m >= 3 ? ring_area(out, m) : 0.0;
@@ -337,6 +355,7 @@ static void *build_worker(void *arg) {
                     ebuf[ne].col = (int32_t)s;
                     ebuf[ne].val = a;
                     ne++;
+                    local_fl_kept += local_fl - fl0;
                 }
             }
             qsort(ebuf, ne, sizeof(entry_t), cmp_entry);
@@ -348,6 +367,9 @@ static void *build_worker(void *arg) {
         }
     }
     atomic_fetch_add(&c->n_candidates, local_cand);
+    atomic_fetch_add(&c->clip_flops, local_fl);
+    atomic_fetch_add(&c->clip_flops_kept, local_fl_kept);
+    g_fl = NULL;
     free(cand);
     free(ebuf);
     return NULL;
@@ -448,6 +470,8 @@ orc_result *orc_build(const double *dst_xyz, int64_t n_dst, int32_t dst_nv,
         free(c.row_entries[d]);
     }
     R->n_candidates = atomic_load(&c.n_candidates);
+    R->clip_flops = atomic_load(&c.clip_flops);
+    R->clip_flops_kept = atomic_load(&c.clip_flops_kept);
     R->status = atomic_load(&c.failed) ? 2 : 0;
     free(c.row_entries); free(c.row_count); free(g.sorted); free(dcaps); free(scaps);
     double t2 = now_s();
@@ -458,6 +482,19 @@ orc_result *orc_build(const double *dst_xyz, int64_t n_dst, int32_t dst_nv,
 
 int64_t orc_result_nnz(const orc_result *R) { return R->nnz; }
 int64_t orc_result_candidates(const orc_result *R) { return R->n_candidates; }
+int64_t orc_result_clip_flops(const orc_result *R) { return R->clip_flops; }
+int64_t orc_result_clip_flops_kept(const orc_result *R) { return R->clip_flops_kept; }
+/* Count the FP64 operations of later builds' clips (on: 1) or not (0, the default). */
+void orc_set_flop_counting(int32_t on) { g_count_flops = on != 0; }
+/* FP64 operations of orc_clip_area on one pair (the same count a build adds per candidate). */
+int64_t orc_clip_area_flops(const double *subj, int32_t ns, const double *clip, int32_t nc) {
+    int64_t fl = 0;
+    int64_t *saved = g_fl;
+    g_fl = &fl;
+    (void)orc_clip_area(subj, ns, clip, nc);
+    g_fl = saved;
+    return fl;
+}
 int32_t orc_result_status(const orc_result *R) { return R->status; }
 void orc_result_times(const orc_result *R, double *out3) {
     out3[0] = R->t_total; out3[1] = R->t_areas; out3[2] = R->t_pairs;

@@ -21,6 +21,7 @@ for the file:line map):
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 import subprocess
 from dataclasses import dataclass
@@ -60,9 +61,13 @@ def lib():
     L.orc_build.restype = ctypes.c_void_p
     L.orc_build.argtypes = [_dp, ctypes.c_int64, ctypes.c_int32, _dp, ctypes.c_int64, ctypes.c_int32,
                             ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64]
-    for name in ("orc_result_nnz", "orc_result_candidates"):
+    for name in ("orc_result_nnz", "orc_result_candidates", "orc_result_clip_flops", "orc_result_clip_flops_kept"):
         getattr(L, name).restype = ctypes.c_int64
         getattr(L, name).argtypes = [ctypes.c_void_p]
+    L.orc_set_flop_counting.restype = None
+    L.orc_set_flop_counting.argtypes = [ctypes.c_int32]
+    L.orc_clip_area_flops.restype = ctypes.c_int64
+    L.orc_clip_area_flops.argtypes = [_dp, ctypes.c_int32, _dp, ctypes.c_int32]
     L.orc_result_status.restype = ctypes.c_int32
     L.orc_result_status.argtypes = [ctypes.c_void_p]
     L.orc_result_times.restype = None
@@ -121,6 +126,13 @@ def polygon_area(ring) -> float:
     """Signed solid angle of an open ring of unit vectors, shape (n, 3)."""
     r = _f64(ring)
     return lib().orc_polygon_area(_d(r), r.shape[0])
+
+
+def clip_area_flops(subject, clip) -> int:
+    """FP64 operations clip_area(subject, clip) performs (adds, subs, muls,
+    divs, sqrts; atan2 counts as one)."""
+    S, C = _f64(subject), _f64(clip)
+    return int(lib().orc_clip_area_flops(_d(S), S.shape[0], _d(C), C.shape[0]))
 
 
 def clip_area(subject, clip) -> float:
@@ -230,6 +242,8 @@ class OracleCSR:
     n_candidates: int
     seconds: float
     threads: int
+    clip_flops: int = 0  # FP64 operations of every candidate's clip and area (build(count_flops=True))
+    clip_flops_kept: int = 0  # the same over the candidates kept as non-zeros
 
     @property
     def nnz(self) -> int:
@@ -254,15 +268,21 @@ class OracleCSR:
 
 
 def build(dst_cells, src_cells, clip_is_dst: bool = True, threshold_rel: float = 1e-14,
-          nthreads: int = 0, row_range=None) -> OracleCSR:
+          nthreads: int = 0, row_range=None, count_flops: bool = False) -> OracleCSR:
     """Regridder build on the unit sphere.  ``dst_cells``/``src_cells`` are
-    polygon tables (ncells, nv, 3).  ``clip_is_dst`` picks the convex side."""
+    polygon tables (ncells, nv, 3).  ``clip_is_dst`` picks the convex side.
+    ``count_flops``: also count the FP64 operations of every candidate's clip
+    and area (``OracleCSR.clip_flops``; slower, not for timing)."""
     d, s = _f64(dst_cells), _f64(src_cells)
     nth = nthreads or default_threads()
     r0, r1 = (0, d.shape[0]) if row_range is None else row_range
     L = lib()
-    h = L.orc_build(_d(d), d.shape[0], d.shape[1], _d(s), s.shape[0], s.shape[1],
-                    1 if clip_is_dst else 0, threshold_rel, nth, r0, r1)
+    L.orc_set_flop_counting(1 if count_flops else 0)
+    try:
+        h = L.orc_build(_d(d), d.shape[0], d.shape[1], _d(s), s.shape[0], s.shape[1],
+                        1 if clip_is_dst else 0, threshold_rel, nth, r0, r1)
+    finally:
+        L.orc_set_flop_counting(0)
     try:
         st = L.orc_result_status(h)
         if st != 0:
@@ -278,9 +298,12 @@ def build(dst_cells, src_cells, clip_is_dst: bool = True, threshold_rel: float =
         t = np.zeros(3)
         L.orc_result_times(h, _d(t))
         ncand = L.orc_result_candidates(h)
+        flops = L.orc_result_clip_flops(h)
+        flops_kept = L.orc_result_clip_flops_kept(h)
     finally:
         L.orc_result_free(h)
-    return OracleCSR(d.shape[0], s.shape[0], rowptr, col, val, a_dst, a_src, ncand, float(t[0]), nth)
+    return OracleCSR(d.shape[0], s.shape[0], rowptr, col, val, a_dst, a_src, ncand, float(t[0]), nth, flops,
+                     flops_kept)
 
 
 def spmv(rowptr, col, val, x, area=None, nthreads: int = 1) -> np.ndarray:
@@ -432,3 +455,84 @@ def fv_to_se(elements, cells, nq: int, degree: int | None = None, node_weights=N
     np.add.at(rp, r + 1, 1)
     rp = np.cumsum(rp)
     return OracleCSR(n, C.shape[0], rp, cc.astype(np.int32), v, np.ones(n), P.a_dst, P.n_candidates, 0.0, 1)
+
+
+# ---------------------------------------------------------------- example fields
+# Restatement of src/utils/example_data.jl (functions of lon, lat in degrees)
+# and of the test helpers' SphericalPolygonIntegrator / set_field_values!
+# (lib/ConservativeRegriddingTestHelpers/src/ConservativeRegriddingTestHelpers.jl:84-128).
+
+def _rad(d):
+    return np.asarray(d, np.float64) * (np.pi / 180.0)
+
+
+def oracle_field(name: str, lon, lat, **kw):
+    """The example field ``name`` (longitude, sinusoid, harmonic, gulf_stream,
+    vortex) at (lon, lat) degrees, with the reference's keyword defaults."""
+    lon, lat = np.broadcast_arrays(np.asarray(lon, np.float64), np.asarray(lat, np.float64))
+    if name == "longitude":
+        return lon.copy()
+    if name in ("sinusoid", "gulf_stream"):
+        dp = kw.get("dp_length", 1.2 * np.pi)
+        coef = kw.get("coef", 2.0)
+        base = coef - np.cos(np.pi * np.arccos(np.cos(_rad(lat)) * np.cos(_rad(lon))) / dp)
+        if name == "sinusoid":
+            return kw.get("coefmult", 1.0) * base
+        ori = (kw.get("gf_ori_lon", -80.0), kw.get("gf_ori_lat", 25.0))
+        end = (kw.get("gf_end_lon", -1.8), kw.get("gf_end_lat", 50.0))
+        dmp = (kw.get("gf_dmp_lon", -25.5), kw.get("gf_dmp_lat", 55.5))
+        dr0 = math.sqrt(float(_rad(end[0] - ori[0])) ** 2 + float(_rad(end[1] - ori[1])) ** 2)
+        dr1 = math.sqrt(float(_rad(dmp[0] - ori[0])) ** 2 + float(_rad(dmp[1] - ori[1])) ** 2)
+        wrapped = lon.copy()
+        wrapped[lon > 180.0] -= 360.0
+        wrapped[lon < -180.0] += 360.0
+        dx, dy = _rad(wrapped - ori[0]), _rad(lat - ori[1])
+        dr = np.sqrt(dx * dx + dy * dy)
+        dth = np.arctan2(dy, dx)
+        dc = np.full(lon.shape, 1.3 * kw.get("gf_coef", 1.0))
+        mid = (dr > dr1) & (dr <= dr0)
+        dc[mid] *= np.cos(0.5 * np.pi * (dr[mid] - dr1) / (dr0 - dr1))
+        dc[dr > dr0] = 0.0
+        term = np.maximum(1000.0 * np.sin(0.4 * (0.5 * dr + dth) + 0.007 * np.cos(50.0 * dth) + 0.37 * np.pi), 999.0)
+        return base + (term - 999.0) * dc
+    if name == "harmonic":
+        return 2.0 + np.sin(2.0 * _rad(lat)) ** 16 * np.cos(16.0 * _rad(lon))
+    if name == "vortex":
+        lon0, lat0 = kw.get("lon0_rad", 5.5), kw.get("lat0_rad", 0.2)
+        r0, d, t = kw.get("r0", 3.0), kw.get("d", 5.0), kw.get("t", 6.0)
+        # rotate so that (lon0, lat0) becomes the pole
+        phi, lam = _rad(lat), _rad(lon) - lon0
+        x = math.sin(lat0) * np.cos(phi) * np.cos(lam) - math.cos(lat0) * np.sin(phi)
+        y = np.cos(phi) * np.sin(lam)
+        z = math.sin(lat0) * np.sin(phi) + math.cos(lat0) * np.cos(phi) * np.cos(lam)
+        rlon = np.arctan2(y, x)
+        rlon = np.where(rlon < 0.0, rlon + 2.0 * np.pi, rlon)
+        rho = r0 * np.cos(np.arcsin(np.clip(z, -1.0, 1.0)))
+        vt = 1.5 * math.sqrt(3.0) * np.tanh(rho) / np.cosh(rho) ** 2
+        omega = np.divide(vt, rho, out=np.zeros_like(rho), where=rho != 0.0)
+        return 2.0 * (1.0 + np.tanh(rho / d * np.sin(rlon - omega * t)))
+    raise ValueError(name)
+
+
+def cell_average(ring, f, degree: int = 8, parts: bool = False):
+    """int f dOmega / int dOmega over one cell (open ring of unit vectors, (n, 3))
+    by the fan rule of the test helpers: f(lon_deg, lat_deg) vectorised; 0 for a
+    cell of zero solid angle.  parts: (int f dOmega, int dOmega) instead."""
+    bary, w = dunavant(degree)
+    r = _f64(ring)
+    A = r[0]
+    num = den = 0.0
+    for i in range(1, r.shape[0] - 1):
+        B, C = r[i], r[i + 1]
+        det = abs(float(np.dot(A, np.cross(B, C))))
+        p = bary[:, :1] * A + bary[:, 1:2] * B + bary[:, 2:3] * C
+        npn = np.linalg.norm(p, axis=1)
+        s = p / npn[:, None]
+        J = det / npn ** 3
+        lon = np.degrees(np.arctan2(s[:, 1], s[:, 0]))
+        lat = np.degrees(np.arcsin(np.clip(s[:, 2], -1.0, 1.0)))
+        num += float(np.sum(w * f(lon, lat) * J))
+        den += float(np.sum(w * J))
+    if parts:
+        return num, den
+    return 0.0 if den == 0.0 else num / den
