@@ -22,7 +22,11 @@
 //        (host copies of CSR(A) sort each row's columns, like sparse())
 //   areas(manifold, tree)  -> the k_cell_bounds areas, times radius^2
 //   normalize!             -> k_scale by 1 / max(A)
-#include <hipcub/hipcub.hpp>
+#include <limits>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_reduce.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -946,20 +950,20 @@ struct Timer {
 
 template <class In, class Out>
 void exclusive_sum(Temp &T, In in, Out out, int64_t n, hipStream_t s) {
+    using V = typename std::iterator_traits<In>::value_type;
     size_t bytes = 0;
-    CR_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, s));
+    CR_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, V(0), (size_t)n, rocprim::plus<V>(), s));
     void *p = T.get(bytes);
-    CR_HIP(hipcub::DeviceScan::ExclusiveSum(p, bytes, in, out, (int)n, s));
+    CR_HIP(rocprim::exclusive_scan(p, bytes, in, out, V(0), (size_t)n, rocprim::plus<V>(), s));
 }
 
 template <class K, class V>
 void sort_pairs(Temp &T, const K *kin, K *kout, const V *vin, V *vout, int64_t n, int end_bit,
                 hipStream_t s) {
     size_t bytes = 0;
-    CR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, (int)n, 0,
-                                              end_bit, s));
+    CR_HIP(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, (size_t)n, 0u, (unsigned)end_bit, s));
     void *p = T.get(bytes);
-    CR_HIP(hipcub::DeviceRadixSort::SortPairs(p, bytes, kin, kout, vin, vout, (int)n, 0, end_bit, s));
+    CR_HIP(rocprim::radix_sort_pairs(p, bytes, kin, kout, vin, vout, (size_t)n, 0u, (unsigned)end_bit, s));
 }
 
 // Instantiated here for the planner (plan.hip, planblocks.hip) too: their code
@@ -1221,9 +1225,10 @@ void normalize_regridder(cr_regridder *R, hipStream_t s) {
     Temp T(s);
     DevBuf<double> mx(1, s);
     size_t bytes = 0;
-    CR_HIP(hipcub::DeviceReduce::Max(nullptr, bytes, F.val.p, mx.p, (int)F.nnz, s));
+    const double lowest = -std::numeric_limits<double>::infinity();
+    CR_HIP(rocprim::reduce(nullptr, bytes, F.val.p, mx.p, lowest, (size_t)F.nnz, rocprim::maximum<double>(), s));
     void *p = T.get(bytes);
-    CR_HIP(hipcub::DeviceReduce::Max(p, bytes, F.val.p, mx.p, (int)F.nnz, s));
+    CR_HIP(rocprim::reduce(p, bytes, F.val.p, mx.p, lowest, (size_t)F.nnz, rocprim::maximum<double>(), s));
     double m = 0;
     CR_HIP(hipMemcpyAsync(&m, mx.p, sizeof(double), hipMemcpyDeviceToHost, s));
     CR_HIP(hipStreamSynchronize(s));

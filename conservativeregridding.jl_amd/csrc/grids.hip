@@ -123,9 +123,9 @@ void orient_grid(cr_grid *g, hipStream_t s) {
 }
 
 template <class F>
-void parallel_for(int64_t n, F &&f) {
+void parallel_for(int64_t n, F &&f, int64_t work_per_item = 1) {
     unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    if (n < 4096) nt = 1;
+    if (n * work_per_item < 4096) nt = 1;
     std::vector<std::thread> th;
     int64_t chunk = (n + nt - 1) / nt;
     for (unsigned t = 0; t < nt; ++t) {
@@ -213,19 +213,109 @@ void validate_cells(int64_t ncells, int nv, bool convex, Get get, const char *wh
     if (collapsed) *collapsed = ncoll.load();
 }
 
-// The cells of nblocks point matrices ((ni+1) x (nj+1), i fastest), in the
-// CellBasedGrid vertex order of src/trees/grids.jl:71-84.
+// Geometric checks of one cell of a point block (corners in CellBasedGrid
+// order): an edge between antipodal points (*anti), no edge at all (all four
+// corners coincide: a legal zero-area cell, *collapsed), or a corner outside
+// an edge by more than kConvexTol (*nonconvex).  The same tests as
+// validate_cells, for host and device.
+CR_HD void check_quad(const d3 (&v)[4], bool *anti, bool *collapsed, bool *nonconvex) {
+    bool any_edge = false;
+    double smin = 0.0, smax = 0.0;
+    *anti = *collapsed = *nonconvex = false;
+    for (int k = 0; k < 4; ++k) {
+        const d3 a = v[k], e = v[(k + 1) & 3] - a;
+        if (dot(e, e) < kCoincident2) continue;  // a repeated vertex (pole corner)
+        any_edge = true;
+        const d3 sum = a + v[(k + 1) & 3];
+        if (dot(sum, sum) < kCoincident2) {
+            *anti = true;
+            return;
+        }
+        const d3 n = cross(a, e);
+        const double nn = dot(n, n);
+        if (nn < kDegenerateEdge2) continue;
+        const double inv = 1.0 / sqrt(nn);
+        for (int q = 0; q < 4; ++q) {
+            const double sq = dot(n, v[q] - a) * inv;
+            smin = fmin(smin, sq);
+            smax = fmax(smax, sq);
+        }
+    }
+    *collapsed = !any_edge;
+    *nonconvex = smin < -kConvexTol && smax > kConvexTol;
+}
+
+CR_HD void point_cell(const double *pts, int64_t ni, int64_t nj, int64_t c, d3 (&v)[4]) {
+    const int64_t W = ni + 1, per = ni * nj, b = c / per, r = c - b * per, i = r % ni, j = r / ni;
+    const double *p0 = pts + 3 * (b * W * (nj + 1) + j * W), *p1 = p0 + 3 * W;
+    v[0] = load3(p0 + 3 * i);
+    v[1] = load3(p0 + 3 * (i + 1));
+    v[2] = load3(p1 + 3 * (i + 1));
+    v[3] = load3(p1 + 3 * i);
+}
+
+// Every point of nblocks point matrices ((ni+1) x (nj+1), i fastest) a finite
+// unit vector, each tested once; a bad point is reported as the lowest cell it
+// belongs to.
+void validate_point_units(const double *pts, int64_t nblocks, int64_t ni, int64_t nj, const char *what) {
+    const int64_t W = ni + 1, H = nj + 1, per = ni * nj;
+    std::atomic<int64_t> bad_unit{INT64_MAX};
+    parallel_for(nblocks * H, [&](int64_t row) {
+        const int64_t b = row / H, j = row - b * H;
+        const double *p = pts + 3 * row * W;
+        for (int64_t i = 0; i < W; ++i) {
+            const d3 a = load3(p + 3 * i);
+            if (!(fabs(dot(a, a) - 1.0) <= kUnitTol)) {  // also false for NaN or infinite coordinates
+                const int64_t ci = std::min(std::max<int64_t>(i - 1, 0), ni - 1);
+                const int64_t cj = std::min(std::max<int64_t>(j - 1, 0), nj - 1);
+                first(bad_unit, b * per + ci + cj * ni);
+            }
+        }
+    }, W);
+    CR_REQUIRE(bad_unit.load() == INT64_MAX, CR_INVALID_ARGUMENT,
+               std::string(what) + ": cell " + std::to_string(bad_unit.load()) +
+                   " has a vertex that is not a finite unit vector (grids are on the unit sphere)");
+}
+
+void raise_point_cells(const char *what, int64_t anti, int64_t nonconvex) {
+    CR_REQUIRE(anti == INT64_MAX, CR_DEGENERATE_GEOMETRY,
+               std::string(what) + ": cell " + std::to_string(anti) +
+                   " has an edge between antipodal points (its great circle is undefined)");
+    CR_REQUIRE(nonconvex == INT64_MAX, CR_DEGENERATE_GEOMETRY,
+               std::string(what) + ": cell " + std::to_string(nonconvex) +
+                   " is not convex but the grid is declared convex (the clip needs a convex side)");
+}
+
+// The checks of validate_cells for the cells of point blocks, on the host.
 void validate_point_blocks(const double *pts, int64_t nblocks, int64_t ni, int64_t nj, const char *what,
                            int64_t *collapsed) {
-    const int64_t W = ni + 1, per = ni * nj;
-    validate_cells(
-        nblocks * per, 4, true,
-        [&](int64_t c, int k) {
-            const int64_t b = c / per, r = c - b * per, i = r % ni, j = r / ni;
-            const int64_t di = (k == 1 || k == 2) ? 1 : 0, dj = (k >= 2) ? 1 : 0;
-            return pts + 3 * (b * W * (nj + 1) + (i + di) + (j + dj) * W);
-        },
-        what, collapsed);
+    validate_point_units(pts, nblocks, ni, nj, what);
+    std::atomic<int64_t> bad_anti{INT64_MAX}, bad_convex{INT64_MAX}, ncoll{0};
+    parallel_for(nblocks * ni * nj, [&](int64_t c) {
+        d3 v[4];
+        point_cell(pts, ni, nj, c, v);
+        bool anti, coll, nonconvex;
+        check_quad(v, &anti, &coll, &nonconvex);
+        if (anti) first(bad_anti, c);
+        else if (nonconvex) first(bad_convex, c);
+        if (!anti && coll) ncoll.fetch_add(1);
+    });
+    raise_point_cells(what, bad_anti.load(), bad_convex.load());
+    if (collapsed) *collapsed = ncoll.load();
+}
+
+// check_quad for every cell of uploaded point blocks: res[0] / res[1] the
+// lowest cell with an antipodal edge / that is not convex (all bits set: none).
+__global__ void k_check_point_cells(const double *__restrict__ pts, int64_t nblocks, int64_t ni, int64_t nj,
+                                    unsigned long long *__restrict__ res) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nblocks * ni * nj) return;
+    d3 v[4];
+    point_cell(pts, ni, nj, c, v);
+    bool anti, coll, nonconvex;
+    check_quad(v, &anti, &coll, &nonconvex);
+    if (anti) atomicMin(res, (unsigned long long)c);
+    else if (nonconvex) atomicMin(res + 1, (unsigned long long)c);
 }
 
 // Vertex count of polygon c of a ragged list; a ring whose last index repeats
@@ -474,12 +564,26 @@ cr_status cr_grid_from_points(cr_grid **grid, const double *pts, int64_t nblocks
         CR_REQUIRE(grid && pts && nblocks > 0 && ni > 0 && nj > 0, CR_INVALID_ARGUMENT,
                    "cr_grid_from_points: bad arguments");
         CR_REQUIRE(nblocks * ni * nj < (int64_t(1) << 31), CR_UNSUPPORTED, "more than 2^31-1 cells");
-        validate_point_blocks(pts, nblocks, ni, nj, "cr_grid_from_points", nullptr);
+        // the points on the host, the cells' geometry on the device (the same
+        // tests as cr_validate_points, without a host pass over every cell)
+        validate_point_units(pts, nblocks, ni, nj, "cr_grid_from_points");
         check_device();
         hipStream_t s = as_stream(stream);
         int64_t npts = nblocks * (ni + 1) * (nj + 1);
         DevBuf<double> dpts((size_t)npts * 3);
         CR_HIP(hipMemcpyAsync(dpts.p, pts, dpts.bytes(), hipMemcpyHostToDevice, s));
+        {
+            DevBuf<unsigned long long> res(2, s);
+            CR_HIP(hipMemsetAsync(res.p, 0xff, res.bytes(), s));
+            hipLaunchKernelGGL(k_check_point_cells, dim3(grid_for(nblocks * ni * nj, 256)), dim3(256), 0, s, dpts.p,
+                               nblocks, ni, nj, res.p);
+            CR_HIP(hipGetLastError());
+            unsigned long long h[2] = {0, 0};
+            CR_HIP(hipMemcpyAsync(h, res.p, sizeof(h), hipMemcpyDeviceToHost, s));
+            CR_HIP(hipStreamSynchronize(s));
+            auto cell = [](unsigned long long v) { return v == ~0ull ? INT64_MAX : (int64_t)v; };
+            raise_point_cells("cr_grid_from_points", cell(h[0]), cell(h[1]));
+        }
         auto *g = new cr_grid();
         try {
             g->ncells = nblocks * ni * nj;

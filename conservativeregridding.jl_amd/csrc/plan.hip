@@ -24,7 +24,7 @@
 //
 // The huge rows (more than kCols non-zeros) are listed by a flag scan.  The
 // device-wide sort and scan come from build.hip (sort_pairs, exclusive_sum).
-#include <hipcub/backend/rocprim/block/block_radix_sort.hpp>
+#include <rocprim/block/block_radix_sort.hpp>
 
 #include "regridder.hpp"
 
@@ -335,10 +335,10 @@ __global__ void __launch_bounds__(G::kThreads) k_plan_fill(const TileRec *__rest
                                                            uint16_t *__restrict__ rseg, int *__restrict__ err,
                                                            unsigned long long *__restrict__ stats) {
     constexpr int NT = G::kThreads, RPT = G::kRows / NT, IPT = G::kEnt, ZMAX = NT * IPT;
-    using Sort = hipcub::BlockRadixSort<uint32_t, NT, IPT, uint16_t>;
+    using Sort = rocprim::block_radix_sort<uint32_t, NT, IPT, uint16_t>;
     // the sort's storage is free once every thread is past the barrier after it
     __shared__ union {
-        typename Sort::TempStorage sort;
+        typename Sort::storage_type sort;
         uint16_t lc[ZMAX];  // local column of each non-zero
     } u;
     uint16_t *lc = u.lc;
@@ -410,7 +410,7 @@ __global__ void __launch_bounds__(G::kThreads) k_plan_fill(const TileRec *__rest
             key[i] = (uint32_t)col[rk[lr] + (q - qs[lr])];
         }
     }
-    Sort(u.sort).Sort(key, qv, 0, col_bits);
+    Sort().sort(key, qv, u.sort, 0, col_bits);
     lastkey[tid] = key[IPT - 1];
     __syncthreads();
     uint32_t fl[1] = {0u}, ftot[1];

@@ -90,7 +90,7 @@ void finalize_regridder(cr_regridder *R, hipStream_t s);
 // The device build of Regridder(dst, src) into R (no normalize!).
 void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src, const cr_build_opts &o,
                      hipStream_t s);
-// hipcub temporary storage, grown on demand.
+// Temporary storage of the device-wide rocPRIM algorithms, grown on demand.
 struct Temp {
     hipStream_t s;
     DevBuf<unsigned char> buf;
@@ -100,7 +100,7 @@ struct Temp {
         return buf.p;
     }
 };
-// Device-wide hipcub exclusive sum and stable radix sort of (key, value) pairs
+// Device-wide rocPRIM exclusive sum and stable radix sort of (key, value) pairs
 // on the bits [0, end_bit), with T's temporary storage (build.hip).
 template <class In, class Out>
 void exclusive_sum(Temp &T, In in, Out out, int64_t n, hipStream_t s);

@@ -179,3 +179,35 @@ def test_zero_area_cells_are_counted(cr, orc):
     assert rp[-1] == rp[-2]  # the collapsed cell's row is empty
     ref = orc.build(dcells, cr.points_to_cells(sp_))
     pattern_and_values(R, ref)
+
+
+def test_point_grid_cells_are_checked_on_the_device(cr):
+    """Grid.from_points checks the points on the host and the cells' geometry
+    on the device: the same errors, naming the same lowest cell, as the host
+    check cr_validate_points."""
+    import ctypes
+    import re
+    _print_case("device_point_checks")
+    L = cr._lib
+
+    def host(P):
+        c, nb, ni, nj = cr.grids._point_blocks(P)
+        st = L.lib.cr_validate_points(L.dptr(c), nb, ni, nj, ctypes.byref(ctypes.c_int64()))
+        return st, L.last_error()
+
+    base = cr.lonlat_points(36, 18)
+    anti = base.copy()
+    anti[20, 9] = -anti[21, 9]  # an edge between antipodal points
+    concave = base.copy()
+    v = concave[7, 5] + 0.8 * (concave[8, 6] - concave[7, 5])  # a corner pulled across its cell
+    concave[7, 5] = v / np.linalg.norm(v)
+    for P in (anti, concave):
+        st, msg = host(P)
+        assert st == L.CR_DEGENERATE_GEOMETRY
+        with pytest.raises(cr.CRError) as e:
+            cr.Grid.from_points(P)
+        assert e.value.status == L.CR_DEGENERATE_GEOMETRY
+        cell = lambda m: re.search(r"cell (\d+)", m).group(1)  # noqa: E731
+        assert cell(msg) == cell(str(e.value))
+    assert host(base)[0] == 0
+    cr.Grid.from_points(base)
