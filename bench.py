@@ -435,11 +435,12 @@ def report_configs(cr, torch, reps, with_cpu):
     # ---- HEALPix nside 1024 -> 0.1 deg lon-lat: 64 FP32 fields (harmonics of
     # degree <= 256 + 1 % noise, synthesised on the device), forward and transposed
     hsrc = cr.Grid.healpix(1024, "nested", step=1)
-    hcells = hsrc.cells()
+    hcells = hsrc.cells()  # host table: the CPU leg's input and the field synthesis' centres
     del hsrc
     dp01 = cr.lonlat_points(3600, 1800)
+    # host inputs: the destination point matrix and the HEALPix nside (cells made on the device)
     R, dst, src, ms = build_from_host(lambda: cr.Grid.from_points(dp01),
-                                      lambda: cr.Grid.from_cells(hcells, convex=True), reps_b=1)
+                                      lambda: cr.Grid.healpix(1024, "nested", step=1), reps_b=1)
     ns, nd, k = src.ncells, dst.ncells, 64
     log(f"[configs] HEALPix 1024 -> 0.1deg: build {ms:.1f} ms from host arrays, nnz {R.nnz}")
     centres = hcells.sum(axis=1)
@@ -592,34 +593,6 @@ def main():
     log(f"[bench] build from device-resident grids: nnz={nnz} candidates={ncand} wall={build_s * 1e3:.1f} ms "
         f"(stages ms: bounds {st['ms_bounds']:.2f} tree {st['ms_tree']:.2f} search {st['ms_search']:.2f} "
         f"clip {st['ms_clip']:.2f} assemble {st['ms_assemble']:.2f})")
-    # the same build from the host inputs of the reference's call: the source
-    # point matrix (uploaded and checked) and the HEALPix nside (its 16-gons
-    # generated on the device, as the reference's HealpixRootNode makes its
-    # cells); and from host vertex tables of both grids (the destination's
-    # 302 MB of 16-gons uploaded and checked too)
-    dst_cells_host = dst.cells()
-
-    def timed_builds(make):
-        ts = []
-        for i in range(args.builds + 1):
-            torch.cuda.synchronize()
-            tb = time.perf_counter()
-            Rh = make()
-            torch.cuda.synchronize()
-            if i:
-                ts.append(time.perf_counter() - tb)
-            assert Rh.nnz == nnz, (Rh.nnz, nnz)
-            del Rh
-        return float(np.median(ts))
-
-    build_host_s = timed_builds(lambda: cr.Regridder(cr.Grid.healpix(NSIDE, "nested", step=STEP),
-                                                     cr.Grid.from_points(src_pts)))
-    build_tables_s = timed_builds(lambda: cr.Regridder(cr.Grid.from_cells(dst_cells_host, convex=False),
-                                                       cr.Grid.from_points(src_pts)))
-    log(f"[bench] build from host inputs (source points, HEALPix nside): {build_host_s * 1e3:.1f} ms "
-        f"({nnz / build_host_s / 1e9:.3f} G pairs/s); from host vertex tables of both grids: "
-        f"{build_tables_s * 1e3:.1f} ms")
-
     # ---- fields --------------------------------------------------------------
     n_dst, n_src = R.shape
     X2 = synth_lonlat_fields(torch, NLON, NLAT, K, 64, seed=0)  # (256, n_src) = Julia [n, 64, 4]
@@ -647,6 +620,36 @@ def main():
     del R3
     log(f"[bench] first batched apply (with its one-time tile plan): {first_apply_ms:.1f} ms in this process, "
         f"{first_apply2_ms:.1f} ms for the next regridder")
+    # the same build from the host inputs of the reference's call: the source
+    # point matrix (uploaded and checked) and the HEALPix nside (its 16-gons
+    # generated on the device, as the reference's HealpixRootNode makes its
+    # cells); and from host vertex tables of both grids (the destination's
+    # 302 MB of 16-gons uploaded and checked too)
+    dst_cells_host = dst.cells()
+
+    def timed_builds(make_grids):
+        """Median wall time from the host inputs to the regridder; the grids and
+        the regridder are released after the clock stops."""
+        ts = []
+        for i in range(args.builds + 1):
+            torch.cuda.synchronize()
+            tb = time.perf_counter()
+            g_dst, g_src = make_grids()
+            Rh = cr.Regridder(g_dst, g_src)
+            torch.cuda.synchronize()
+            if i:
+                ts.append(time.perf_counter() - tb)
+            assert Rh.nnz == nnz, (Rh.nnz, nnz)
+            del Rh, g_dst, g_src
+        return float(np.median(ts))
+
+    build_host_s = timed_builds(lambda: (cr.Grid.healpix(NSIDE, "nested", step=STEP), cr.Grid.from_points(src_pts)))
+    build_tables_s = timed_builds(lambda: (cr.Grid.from_cells(dst_cells_host, convex=False),
+                                           cr.Grid.from_points(src_pts)))
+    log(f"[bench] build from host inputs (source points, HEALPix nside): {build_host_s * 1e3:.1f} ms "
+        f"({nnz / build_host_s / 1e9:.3f} G pairs/s); from host vertex tables of both grids: "
+        f"{build_tables_s * 1e3:.1f} ms")
+
     variant_ms = {}
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     for v in range(cr.regrid_variants("float32")):
