@@ -26,10 +26,15 @@
 // stay in its L2) and the chunks of one tile run side by side (the tile's
 // entries are read from HBM once).
 //
-// Numerics match the row kernels: FP32 fields use the FP64 A rounded to FP32,
-// FP32 products and Kahan-compensated segment sums, segments added in FP64 and
-// scaled by the FP64 reciprocal area; FP64 fields use fused multiply-adds and a
-// true division by the area.
+// Numerics (MODE of the kernel, chosen per variant in apply.hip): FP32 fields
+// either take products and segment sums in FP64 from the FP64 A (MODE 1, the
+// default) or from A rounded to FP32 (MODE 2), or FP32 products of A rounded to
+// FP32 with Kahan-compensated FP32 segment sums (MODE 0); in every mode the
+// segments of a row are added in FP64, scaled by the FP64 reciprocal area and
+// rounded to FP32 once.  FP64 fields use fused multiply-adds and a true
+// division by the area.  Results are bitwise reproducible for a given variant
+// (a fixed summation order, no atomics); another variant can differ in the
+// last bits.
 #include <algorithm>
 #include <type_traits>
 

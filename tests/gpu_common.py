@@ -31,6 +31,15 @@ def pattern_and_values(R, ref, tol=1e-12, thr=1e-14):
     scale = np.minimum(a_dst[rows[ia]], a_src[col[ia]])
     err = np.max(np.abs(val[ia] - ref.val[ib]) / scale) if ia.size else 0.0
     assert err <= tol, f"intersection areas differ: {err:.3e}"
+    # each pair's error relative to its own area: over every common pair, and
+    # over the pairs of at least 1e-4 of the smaller cell (slivers below that
+    # carry their cells' absolute rounding)
+    if ia.size:
+        rel = np.abs(val[ia] - ref.val[ib]) / ref.val[ib]
+        big = ref.val[ib] >= 1e-4 * scale
+        print(f"[areas] {ia.size} pairs: max error {err:.2e} of the smaller cell; per pair {rel.max():.2e} "
+              f"(all), {rel[big].max() if big.any() else 0.0:.2e} (pairs >= 1e-4 of the smaller cell)", flush=True)
+        assert not big.any() or rel[big].max() <= 1e-8
     np.testing.assert_allclose(R.dst_areas, a_dst, rtol=1e-12)
     np.testing.assert_allclose(R.src_areas, a_src, rtol=1e-12)
     return err, flips
