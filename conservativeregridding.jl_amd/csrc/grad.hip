@@ -57,7 +57,9 @@ struct VertexSink {
     double *own_clip;                  // the pair's own 4 x 3 clip-slot sums in LDS, or null
     double *own_subj;                  // the pair's own 4 x 3 subject-slot sums in LDS, or null
 
-    __device__ d3 sv(int e) const { return load3(subj + 3 * (e % ns)); }
+    // slot e of the subject, 0 <= e <= ns (an edge's end vertex e + 1 wraps)
+    __device__ int wrap(int e) const { return e >= ns ? e - ns : e; }
+    __device__ d3 sv(int e) const { return load3(subj + 3 * wrap(e)); }
     __device__ d3 cv(int k) const {
         int kk = k % 4;
         return load3(clip + 3 * (kk < nc ? kk : nc - 1));
@@ -74,7 +76,7 @@ struct VertexSink {
         atomicAdd(l + 3 * slot + 2, d.z);
     }
     __device__ void add_subj(int e, const d3 &g) const {
-        e %= ns;
+        e = wrap(e);
         if (lacc_subj) {
             add_local(lacc_subj, e, tangential(g, sv(e)));
         } else if (own_subj) {  // this pair's own sums (no other thread touches them)
@@ -217,6 +219,52 @@ __global__ void k_area_grad(const double *__restrict__ cells, int nv, const int3
     });
 }
 
+// The same for grids whose cells own their vertices (private_v) with at most
+// kStagedNv vertices per cell: a workgroup's 64 cells are one contiguous run of
+// vertex data and of g (each cell's caller vertices are its own slots, in some
+// order), so both are staged through LDS with coalesced loads and stores
+// instead of a thread per cell reading and writing 24*nv bytes apart from its
+// neighbours.
+constexpr int kStagedNv = 16;
+__global__ void __launch_bounds__(64) k_area_grad_private(const double *__restrict__ cells, int nv,
+                                                          const int32_t *__restrict__ vi, int64_t ncells,
+                                                          double kappa, const double *__restrict__ a_to, int64_t nf,
+                                                          const double *__restrict__ y, int64_t yc, int64_t yf,
+                                                          const double *__restrict__ gy, int64_t gc, int64_t gf,
+                                                          double *__restrict__ g) {
+    extern __shared__ double stage[];  // 2 * 64 * nv * 3 doubles: vertices, then g
+    double *vin = stage, *gout = stage + 64 * 3 * nv;
+    const int64_t c0 = (int64_t)blockIdx.x * 64;
+    const int ncell = (int)min((int64_t)64, ncells - c0);
+    const int n = ncell * nv * 3;
+    const double *src = cells + 3 * (int64_t)nv * c0;
+    double *gb = g + 3 * (int64_t)nv * c0;
+    for (int i = threadIdx.x; i < n; i += 64) {
+        vin[i] = src[i];
+        gout[i] = gb[i];
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    const int64_t c = c0 + t;
+    if (t < ncell && a_to[c] != 0.0) {  // zero-area (ghost) cells have no area term
+        double sum = 0.0;
+        for (int64_t f = 0; f < nf; ++f) sum += gy[c * gc + f * gf] * y[c * yc + f * yf];
+        const double coef = -kappa * sum / a_to[c];
+        if (coef != 0.0) {
+            const int32_t *v = vi + (int64_t)nv * c;
+            const int64_t base = (int64_t)nv * c0;
+            ring_area_backward(vin + 3 * nv * t, nv, coef, [&](int slot, const d3 &d) {
+                double *o = gout + 3 * (v[slot] - base);
+                o[0] += d.x;
+                o[1] += d.y;
+                o[2] += d.z;
+            });
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 64) gb[i] = gout[i];
+}
+
 __global__ void k_div_rows(const double *__restrict__ gy, int64_t gc, int64_t gf, int64_t n, int64_t nf,
                            const double *__restrict__ a, double *__restrict__ w) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -307,9 +355,16 @@ static void backward(const cr_regridder *R, const cr_grid *dst, const cr_grid *s
                            R->stats.clip_with_dst, g_dst_verts, g_src_verts);
     double *g_to = transpose ? g_src_verts : g_dst_verts;
     const cr_grid *gto = transpose ? src : dst;
-    if (normalize && g_to)
-        hipLaunchKernelGGL(k_area_grad, dim3(grid_for(n_to, B)), dim3(B), 0, s, gto->cells.p, gto->nv, gto->vidx.p,
-                           n_to, kappa, a_to, nfields, y, yc, yf, gy, gc, gf, g_to, gto->private_vertices);
+    if (normalize && g_to) {
+        if (gto->private_vertices && gto->nv <= kStagedNv)
+            hipLaunchKernelGGL(k_area_grad_private, dim3(grid_for(n_to, 64)), dim3(64),
+                               2 * 64 * 3 * gto->nv * sizeof(double), s, gto->cells.p, gto->nv,
+                               gto->vidx.p, n_to, kappa, a_to, nfields, y, yc, yf, gy, gc, gf, g_to);
+        else
+            hipLaunchKernelGGL(k_area_grad, dim3(grid_for(n_to, B)), dim3(B), 0, s, gto->cells.p, gto->nv,
+                               gto->vidx.p, n_to, kappa, a_to, nfields, y, yc, yf, gy, gc, gf, g_to,
+                               gto->private_vertices);
+    }
     CR_HIP(hipGetLastError());
     if (!ws) CR_HIP(hipStreamSynchronize(s));
 }

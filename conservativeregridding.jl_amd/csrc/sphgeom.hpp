@@ -306,7 +306,7 @@ CR_HD void lab_lines(int32_t l, int ns, int32_t &l1, int32_t &l2) {
     int32_t t = lab_type(l), a = lab_hi(l), b = lab_lo(l);
     if (t == kLabS) {
         int32_t e = l & kLabIndexMask;
-        l1 = (e + ns - 1) % ns;
+        l1 = e == 0 ? ns - 1 : e - 1;
         l2 = e;
     }
     else if (t == kLabC) { l1 = kClipLine | ((b + NC - 1) % NC); l2 = kClipLine | b; }
@@ -342,8 +342,10 @@ struct GradPipe {
     int ns;
     double coef;
     Sink *sink;
-    // area stage
-    d3 w0, wp, g0;
+    // area stage: fan triangles (w0, wp, w); the previous vertex's part from
+    // the last triangle waits in pend until the next triangle completes it,
+    // so each vertex reaches the sink once (the sink is linear in g)
+    d3 w0, wp, g0, pend;
     int32_t l0, lp;
     int n;
 
@@ -352,18 +354,24 @@ struct GradPipe {
             w0 = w;
             l0 = l;
             g0 = mk3(0, 0, 0);
+            pend = mk3(0, 0, 0);
         } else if (n >= 2) {
             d3 a = w0, b = wp, c = w;
             double T = dot(a, cross(b, c));
             double D = 1.0 + dot(a, b) + dot(b, c) + dot(c, a);
             double f = 2.0 / (T * T + D * D);
             g0 = g0 + f * (D * cross(b, c) - T * (b + c));
-            (*sink)(lp, b, (coef * f) * (D * cross(c, a) - T * (a + c)));
-            (*sink)(l, c, (coef * f) * (D * cross(a, b) - T * (a + b)));
+            (*sink)(lp, b, pend + (coef * f) * (D * cross(c, a) - T * (a + c)));
+            pend = (coef * f) * (D * cross(a, b) - T * (a + b));
         }
         wp = w;
         lp = l;
         ++n;
+    }
+    CR_HD void area_close() {
+        if (n < 3) return;
+        (*sink)(lp, wp, pend);
+        (*sink)(l0, w0, coef * g0);
     }
 };
 
@@ -444,7 +452,7 @@ CR_HD void clip_area_backward(const double *clip, int nc, const double *subj, in
     }
     for (int i = 0; i < ns; ++i) gpipe_push<0, NC, Sink>(P, load3(subj + 3 * i), lab_make(kLabS, 0, 0) | i);
     gpipe_close<0, NC, Sink>(P);
-    if (P.n >= 3) sink(P.l0, P.w0, coef * P.g0);
+    P.area_close();
 }
 
 // Gradient w.r.t. the great-circle crossing x = y/|y| (y = +-(n1 x n2),
