@@ -7,7 +7,7 @@ Loaded from its folder by path (the folder name is not a Python identifier):
 from ._lib import CRError, LIB_PATH, lib as _lib_handle
 from .errors import ArgumentError, DefaultIntersectionFailureError, DimensionMismatch, ErrorException, MethodError
 from .manifolds import Geodesic, Planar, Spherical, best_manifold, resolve_manifold
-from .grids import (Grid, NESTED, RING, as_grid, cubed_sphere_points, healpix_cells,
+from .grids import (Grid, NESTED, RING, as_grid, as_planar_grid, cubed_sphere_points, healpix_cells,
                     flatten_tree, healpix_nest2ring, healpix_ring2nest, lonlat_points, lonlat_to_unit, points_to_cells,
                     polygons_to_cells, polygons_width, ring_full_polygons, ring_latitudes, ring_longitudes)
 from .esmf import read_esmf_weights, regridder_from_esmf, save_esmf_weights, write_esmf_arrays
@@ -24,7 +24,7 @@ __all__ = [
     "SpectralElementSpace", "se_moments",
     "ExampleFieldFunction", "LongitudeField", "SinusoidField", "HarmonicField", "GulfStreamField", "VortexField",
     "cell_averages",
-    "Grid", "NESTED", "RING", "as_grid", "lonlat_points", "cubed_sphere_points", "healpix_cells",
+    "Grid", "NESTED", "RING", "as_grid", "as_planar_grid", "lonlat_points", "cubed_sphere_points", "healpix_cells",
     "healpix_nest2ring", "healpix_ring2nest", "points_to_cells", "lonlat_to_unit", "polygons_to_cells",
     "polygons_width", "flatten_tree", "ring_full_polygons", "ring_latitudes", "ring_longitudes",
     "ArgumentError", "DimensionMismatch", "MethodError", "ErrorException", "DefaultIntersectionFailureError",

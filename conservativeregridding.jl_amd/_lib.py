@@ -74,6 +74,9 @@ SIGNATURES = {
     "cr_validate_points": (c_i32, [_dp, c_i64, c_i64, c_i64, _i64p]),
     "cr_grid_from_cells": (c_i32, [ctypes.POINTER(c_vp), _dp, c_i64, c_i32, c_i32, c_vp]),
     "cr_grid_from_polygons": (c_i32, [ctypes.POINTER(c_vp), _dp, c_i64, _i64p, _i32p, c_i64, c_i32, c_vp]),
+    "cr_grid_from_planar_polygons": (c_i32, [ctypes.POINTER(c_vp), _dp, c_i64, _i64p, _i32p, c_i64, c_i32, c_vp]),
+    "cr_grid_is_planar": (c_i32, [c_vp, _i32p]),
+    "cr_grid_copy_planar_cells": (c_i32, [c_vp, _dp]),
     "cr_grid_from_points": (c_i32, [ctypes.POINTER(c_vp), _dp, c_i64, c_i64, c_i64, c_vp]),
     "cr_grid_healpix": (c_i32, [ctypes.POINTER(c_vp), c_i64, c_i32, c_i32, c_vp]),
     "cr_grid_tripolar": (c_i32, [ctypes.POINTER(c_vp), _dp, c_i64, c_i64, c_vp]),

@@ -32,6 +32,7 @@
 #include <chrono>
 #include <vector>
 
+#include "planar.hpp"
 #include "regridder.hpp"
 #include "sphgeom.hpp"
 #include "tree.hpp"
@@ -783,6 +784,26 @@ __global__ void k_clip_generic(const int32_t *__restrict__ cand_dst,
     val[c] = kept_area(a, fmin(a_src[s], a_dst[d]), thr);
 }
 
+// Planar clip + area per candidate (the Planar manifold): both rings gathered
+// from the caller vertices, Sutherland-Hodgman in the plane (planar.hpp).
+// MAXV holds a cell (nv <= MAXV) and bounds the clipped ring.
+template <int MAXV>
+__global__ void k_clip_planar(const int32_t *__restrict__ cand_dst, const int32_t *__restrict__ cand_src,
+                              int64_t ncand, const double *__restrict__ dst_xy, const int32_t *__restrict__ dst_vi,
+                              int dst_nv, const double *__restrict__ src_xy, const int32_t *__restrict__ src_vi,
+                              int src_nv, int clip_is_dst, const double *__restrict__ a_dst,
+                              const double *__restrict__ a_src, double thr, double *__restrict__ val) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncand) return;
+    const int64_t d = cand_dst[c], s = cand_src[c];
+    p2 dr[MAXV], sr[MAXV];
+    for (int k = 0; k < dst_nv; ++k) dr[k] = planar_vertex(dst_xy, dst_vi, d * dst_nv + k);
+    for (int k = 0; k < src_nv; ++k) sr[k] = planar_vertex(src_xy, src_vi, s * src_nv + k);
+    const double a = clip_is_dst ? planar_clip_area<MAXV>(dr, dst_nv, sr, src_nv)
+                                 : planar_clip_area<MAXV>(sr, src_nv, dr, dst_nv);
+    val[c] = kept_area(a, fmin(a_src[s], a_dst[d]), thr);
+}
+
 // ---- wave64 ballot compaction of kept candidates --------------------------
 constexpr int kCompactBlock = 256;
 
@@ -1019,6 +1040,11 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     Timer tm(s);
     Temp T(s);
     tm.mark();  // 0
+    // the Planar manifold needs both grids on it (regridder.jl:105-123 errors on a mismatch)
+    CR_REQUIRE(dst->planar == src->planar, CR_INVALID_ARGUMENT,
+               std::string("Destination and source manifolds must be the same.  Got ") +
+                   (dst->planar ? "Planar() and Spherical()" : "Spherical() and Planar()"));
+    const bool planar = dst->planar != 0;
 
     // clip side: a convex side clips; prefer the destination (GO clips p1=src by p2=dst)
     int clip_is_dst;
@@ -1045,6 +1071,10 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
     DevBuf<float4> cap_dst(nd, s), cap_src(ns, s);
     launch_cell_bounds(dst, a_dst.p, cap_dst.p, s);
     launch_cell_bounds(src, a_src.p, cap_src.p, s);
+    if (planar) {  // caps of the unit-vector images, areas in the plane
+        launch_planar_areas(dst, a_dst.p, s);
+        launch_planar_areas(src, a_src.p, s);
+    }
     hipLaunchKernelGGL(k_count_cells, dim3(grid_for(nd, B)), dim3(B), 0, s, a_dst.p, nd, int64_t(0), acct.p);
     hipLaunchKernelGGL(k_count_cells, dim3(grid_for(ns, B)), dim3(B), 0, s, a_src.p, ns, nd, acct.p);
     CR_HIP(hipGetLastError());
@@ -1120,7 +1150,17 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
 
     // 4. clip + area per candidate
     DevBuf<double> cval(ncand, s);
-    if (ncand) {
+    if (ncand && planar) {
+        if (std::max(dst->nv, src->nv) <= 8)
+            hipLaunchKernelGGL(k_clip_planar<32>, dim3(grid_for(ncand, 64)), dim3(64), 0, s, cand_dst.p, cand_src.p,
+                               ncand, dst->xy.p, dst->vidx.p, dst->nv, src->xy.p, src->vidx.p, src->nv, clip_is_dst,
+                               a_dst.p, a_src.p, o.threshold_rel, cval.p);
+        else
+            hipLaunchKernelGGL(k_clip_planar<2 * kMaxCellVertices>, dim3(grid_for(ncand, 64)), dim3(64), 0, s,
+                               cand_dst.p, cand_src.p, ncand, dst->xy.p, dst->vidx.p, dst->nv, src->xy.p,
+                               src->vidx.p, src->nv, clip_is_dst, a_dst.p, a_src.p, o.threshold_rel, cval.p);
+        CR_HIP(hipGetLastError());
+    } else if (ncand) {
         if (clip_nv <= 4 && o.threshold_rel > 0.0)
             hipLaunchKernelGGL(k_clip_compact<4>, dim3(grid_for(ncand, 64 * kClipChunks)), dim3(64), 0, s,
                                cand_dst.p, cand_src.p, ncand, dst->cells.p, dst->nv, src->cells.p, src->nv,
@@ -1183,8 +1223,8 @@ void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src,
         csr_transpose(F, coo_row.p, R->bwd, T, s);
     }
 
-    // 6. radius^2 scaling, areas
-    const double r2 = o.radius * o.radius;
+    // 6. radius^2 scaling (none on the plane), areas
+    const double r2 = planar ? 1.0 : o.radius * o.radius;
     R->a_dst = std::move(a_dst);
     R->a_src = std::move(a_src);
     if (r2 != 1.0) {

@@ -160,4 +160,9 @@ struct cr_grid {
     int32_t private_vertices = 0;  // 1: no vertex index is shared by two cells (tables of cells)
     cr::DevBuf<int32_t> fold;    // 2 * nfold: (primary, partner) cells of a folded row
     int64_t nfold = 0;
+    // Planar manifold: the caller's vertices (nverts * 2, x and y); `cells`
+    // then holds their gnomonic images on the unit sphere, which only the
+    // candidate search reads (straight edges map to great-circle arcs)
+    int32_t planar = 0;
+    cr::DevBuf<double> xy;
 };

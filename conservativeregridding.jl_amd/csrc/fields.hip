@@ -129,6 +129,7 @@ cr_status cr_cell_averages(const cr_grid *grid, int32_t field, const double *par
                            int32_t degree, double *out, void *stream) {
     return guard([&] {
         CR_REQUIRE(grid && out, CR_INVALID_ARGUMENT, "cr_cell_averages: null argument");
+        CR_REQUIRE(!grid->planar, CR_UNSUPPORTED, "cr_cell_averages: the example fields are functions on the sphere");
         CR_REQUIRE(field >= CR_FIELD_LONGITUDE && field <= CR_FIELD_VORTEX, CR_INVALID_ARGUMENT,
                    "cr_cell_averages: unknown example field " + std::to_string(field));
         FieldSpec F{};

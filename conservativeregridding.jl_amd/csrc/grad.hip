@@ -329,6 +329,8 @@ static void backward(const cr_regridder *R, const cr_grid *dst, const cr_grid *s
                "vertex gradients need a regridder built from grids (cr_regridder_build)");
     CR_REQUIRE(dst && src && dst->ncells == F.nrows && src->ncells == F.ncols, CR_DIMENSION_MISMATCH,
                "cr_regrid_backward: grids do not match the regridder");
+    CR_REQUIRE(!dst->planar && !src->planar, CR_UNSUPPORTED,
+               "vertex gradients are of spherical intersection areas; these grids are planar");
     const int clip_nv = R->stats.clip_with_dst ? dst->nv : src->nv;
     CR_REQUIRE(clip_nv <= 4, CR_UNSUPPORTED, "vertex gradients need clip cells with at most 4 vertices");
     if (normalize) CR_REQUIRE(y, CR_INVALID_ARGUMENT, "cr_regrid_backward: the forward output y is needed");

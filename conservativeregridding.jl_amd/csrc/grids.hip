@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "gridgen.hpp"
+#include "planar.hpp"
 #include "sphgeom.hpp"
 
 namespace cr {
@@ -108,6 +109,22 @@ __global__ void k_cell_areas(const double *__restrict__ cells, int64_t ncells, i
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncells) return;
     area[c] = ring_area(cells + 3 * nv * c, nv);
+}
+
+// Planar cell areas (shoelace) from the caller vertices, in the slot order the
+// orientation pass left.
+__global__ void k_planar_areas(const double *__restrict__ xy, const int32_t *__restrict__ vidx, int64_t ncells,
+                               int nv, double *__restrict__ area) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    p2 r[kMaxCellVertices];
+    for (int k = 0; k < nv; ++k) r[k] = planar_vertex(xy, vidx, c * nv + k);
+    area[c] = planar_ring_area(r, nv);
+}
+
+void launch_planar_areas(const cr_grid *g, double *area, hipStream_t s) {
+    hipLaunchKernelGGL(k_planar_areas, dim3(grid_for(g->ncells, 256)), dim3(256), 0, s, g->xy.p, g->vidx.p,
+                       g->ncells, g->nv, area);
 }
 
 void orient_grid(cr_grid *g, hipStream_t s) {
@@ -520,41 +537,89 @@ cr_status cr_polygons_width(int64_t nverts, const int64_t *offsets, const int32_
     });
 }
 
+// A device grid from a ragged polygon list of unit vectors (checked, then
+// oriented counter-clockwise on the device).  `planar_xy` (may be null): the
+// caller's planar vertices the unit vectors are the images of, kept on the
+// grid for the planar clip and areas.
+cr_grid *grid_from_polygon_list(const double *xyz, int64_t nverts, const int64_t *offsets, const int32_t *verts,
+                                int64_t ncells, int32_t convex, const double *planar_xy, const char *what,
+                                hipStream_t s) {
+    CR_REQUIRE(ncells < (int64_t(1) << 31) && nverts < (int64_t(1) << 31), CR_UNSUPPORTED,
+               "more than 2^31-1 cells or vertices");
+    const int32_t nv = polygons_width(nverts, offsets, verts, ncells);
+    std::vector<double> cells((size_t)ncells * nv * 3);
+    std::vector<int32_t> vidx((size_t)ncells * nv);
+    polygons_fill(xyz, offsets, verts, ncells, nv, cells.data(), vidx.data());
+    const double *cp = cells.data();
+    validate_cells(
+        ncells, nv, convex != 0, [&](int64_t c, int k) { return cp + 3 * (c * nv + k); }, what, nullptr);
+    check_device();
+    auto *g = new cr_grid();
+    try {
+        g->ncells = ncells;
+        g->nv = nv;
+        g->convex = convex ? 1 : 0;
+        g->nverts = nverts;
+        g->cells.alloc(cells.size());
+        g->vidx.alloc(vidx.size());
+        CR_HIP(hipMemcpyAsync(g->cells.p, cells.data(), g->cells.bytes(), hipMemcpyHostToDevice, s));
+        CR_HIP(hipMemcpyAsync(g->vidx.p, vidx.data(), g->vidx.bytes(), hipMemcpyHostToDevice, s));
+        if (planar_xy) {
+            g->planar = 1;
+            g->xy.alloc((size_t)nverts * 2);
+            CR_HIP(hipMemcpyAsync(g->xy.p, planar_xy, g->xy.bytes(), hipMemcpyHostToDevice, s));
+        }
+        CR_HIP(hipStreamSynchronize(s));  // the host tables end with this scope
+        orient_grid(g, s);
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        delete g;
+        throw;
+    }
+    return g;
+}
+
 cr_status cr_grid_from_polygons(cr_grid **grid, const double *xyz, int64_t nverts, const int64_t *offsets,
                                 const int32_t *verts, int64_t ncells, int32_t convex, void *stream) {
     return guard([&] {
         CR_REQUIRE(grid && xyz && offsets && verts && nverts > 0 && ncells > 0, CR_INVALID_ARGUMENT,
                    "cr_grid_from_polygons: bad arguments");
-        CR_REQUIRE(ncells < (int64_t(1) << 31) && nverts < (int64_t(1) << 31), CR_UNSUPPORTED,
-                   "more than 2^31-1 cells or vertices");
-        const int32_t nv = polygons_width(nverts, offsets, verts, ncells);
-        std::vector<double> cells((size_t)ncells * nv * 3);
-        std::vector<int32_t> vidx((size_t)ncells * nv);
-        polygons_fill(xyz, offsets, verts, ncells, nv, cells.data(), vidx.data());
-        const double *cp = cells.data();
-        validate_cells(
-            ncells, nv, convex != 0, [&](int64_t c, int k) { return cp + 3 * (c * nv + k); },
-            "cr_grid_from_polygons", nullptr);
-        check_device();
-        hipStream_t s = as_stream(stream);
-        auto *g = new cr_grid();
-        try {
-            g->ncells = ncells;
-            g->nv = nv;
-            g->convex = convex ? 1 : 0;
-            g->nverts = nverts;
-            g->cells.alloc(cells.size());
-            g->vidx.alloc(vidx.size());
-            CR_HIP(hipMemcpyAsync(g->cells.p, cells.data(), g->cells.bytes(), hipMemcpyHostToDevice, s));
-            CR_HIP(hipMemcpyAsync(g->vidx.p, vidx.data(), g->vidx.bytes(), hipMemcpyHostToDevice, s));
-            CR_HIP(hipStreamSynchronize(s));  // the host tables end with this scope
-            orient_grid(g, s);
-        } catch (...) {
-            (void)hipStreamSynchronize(s);
-            delete g;
-            throw;
+        *grid = grid_from_polygon_list(xyz, nverts, offsets, verts, ncells, convex, nullptr, "cr_grid_from_polygons",
+                                       as_stream(stream));
+    });
+}
+
+cr_status cr_grid_from_planar_polygons(cr_grid **grid, const double *xy, int64_t nverts, const int64_t *offsets,
+                                       const int32_t *verts, int64_t ncells, int32_t convex, void *stream) {
+    return guard([&] {
+        CR_REQUIRE(grid && xy && offsets && verts && nverts > 0 && ncells > 0, CR_INVALID_ARGUMENT,
+                   "cr_grid_from_planar_polygons: bad arguments");
+        double lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
+        for (int64_t v = 0; v < nverts; ++v)
+            for (int d = 0; d < 2; ++d) {
+                const double c = xy[2 * v + d];
+                CR_REQUIRE(std::isfinite(c), CR_INVALID_ARGUMENT,
+                           "cr_grid_from_planar_polygons: vertex " + std::to_string(v) + " is not finite");
+                lo[d] = std::min(lo[d], c);
+                hi[d] = std::max(hi[d], c);
+            }
+        const double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]);
+        const double h = 0.5 * std::max(hi[0] - lo[0], hi[1] - lo[1]);
+        CR_REQUIRE(h > 0.0 && std::isfinite(h), CR_INVALID_ARGUMENT,
+                   "cr_grid_from_planar_polygons: the vertices span no area");
+        // gnomonic images: (x, y) -> ((x - cx) / h, (y - cy) / h, 1) / norm, the
+        // plane tangent at the pole seen from the centre of the sphere; the
+        // extent maps to within 55 degrees of the pole
+        std::vector<double> xyz((size_t)nverts * 3);
+        for (int64_t v = 0; v < nverts; ++v) {
+            const double X = (xy[2 * v] - cx) / h, Y = (xy[2 * v + 1] - cy) / h;
+            const double r = 1.0 / std::sqrt(X * X + Y * Y + 1.0);
+            xyz[3 * v] = X * r;
+            xyz[3 * v + 1] = Y * r;
+            xyz[3 * v + 2] = r;
         }
-        *grid = g;
+        *grid = grid_from_polygon_list(xyz.data(), nverts, offsets, verts, ncells, convex, xy,
+                                       "cr_grid_from_planar_polygons", as_stream(stream));
     });
 }
 
@@ -723,12 +788,37 @@ cr_status cr_grid_copy_cells(const cr_grid *grid, double *cells) {
     });
 }
 
+cr_status cr_grid_is_planar(const cr_grid *grid, int32_t *planar) {
+    return guard([&] {
+        CR_REQUIRE(grid && planar, CR_INVALID_ARGUMENT, "null argument");
+        *planar = grid->planar;
+    });
+}
+
+cr_status cr_grid_copy_planar_cells(const cr_grid *grid, double *cells_xy) {
+    return guard([&] {
+        CR_REQUIRE(grid && cells_xy, CR_INVALID_ARGUMENT, "null argument");
+        CR_REQUIRE(grid->planar, CR_INVALID_ARGUMENT, "cr_grid_copy_planar_cells: not a planar grid");
+        std::vector<double> xy((size_t)grid->nverts * 2);
+        std::vector<int32_t> vi((size_t)grid->ncells * grid->nv);
+        CR_HIP(hipMemcpy(xy.data(), grid->xy.p, grid->xy.bytes(), hipMemcpyDeviceToHost));
+        CR_HIP(hipMemcpy(vi.data(), grid->vidx.p, grid->vidx.bytes(), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < vi.size(); ++k) {
+            cells_xy[2 * k] = xy[2 * (size_t)vi[k]];
+            cells_xy[2 * k + 1] = xy[2 * (size_t)vi[k] + 1];
+        }
+    });
+}
+
 cr_status cr_grid_areas(const cr_grid *grid, double *areas_host) {
     return guard([&] {
         CR_REQUIRE(grid && areas_host, CR_INVALID_ARGUMENT, "null argument");
         DevBuf<double> a(grid->ncells);
-        hipLaunchKernelGGL(k_cell_areas, dim3(grid_for(grid->ncells, 256)), dim3(256), 0, 0,
-                           grid->cells.p, grid->ncells, grid->nv, a.p);
+        if (grid->planar)
+            launch_planar_areas(grid, a.p, 0);
+        else
+            hipLaunchKernelGGL(k_cell_areas, dim3(grid_for(grid->ncells, 256)), dim3(256), 0, 0,
+                               grid->cells.p, grid->ncells, grid->nv, a.p);
         CR_HIP(hipGetLastError());
         CR_HIP(hipMemcpy(areas_host, a.p, a.bytes(), hipMemcpyDeviceToHost));
     });

@@ -90,6 +90,8 @@ void finalize_regridder(cr_regridder *R, hipStream_t s);
 // The device build of Regridder(dst, src) into R (no normalize!).
 void build_regridder(cr_regridder *R, const cr_grid *dst, const cr_grid *src, const cr_build_opts &o,
                      hipStream_t s);
+// Planar cell areas of a planar grid (grids.hip), on stream s.
+void launch_planar_areas(const cr_grid *g, double *area, hipStream_t s);
 // Temporary storage of the device-wide rocPRIM algorithms, grown on demand.
 struct Temp {
     hipStream_t s;

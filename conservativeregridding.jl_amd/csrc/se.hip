@@ -389,6 +389,7 @@ __global__ void k_scale_rowptr(const int64_t *__restrict__ rp, int64_t n, int64_
 namespace {
 
 void check_elements(const cr_grid *elems) {
+    CR_REQUIRE(!elems->planar, CR_UNSUPPORTED, "spectral elements live on the sphere, not on a planar grid");
     CR_REQUIRE(elems->nv == 4, CR_INVALID_ARGUMENT, "spectral elements must be quadrilaterals");
     CR_REQUIRE(elems->flipped == 0, CR_INVALID_ARGUMENT,
                "element corners must run counter-clockwise from (-1,-1): (-1,-1), (1,-1), (1,1), (-1,1)");
@@ -438,6 +439,7 @@ void launch_moments(const cr_regridder *R, const cr_grid *cells, const cr_grid *
 void moments(const cr_regridder *R, const cr_grid *cells, const cr_grid *elems, int nq, int degree, double scale,
              double *B, hipStream_t s, int32_t *pair_cell = nullptr, int32_t *pair_elem = nullptr) {
     check_elements(elems);
+    CR_REQUIRE(!cells->planar, CR_UNSUPPORTED, "spectral-element weights need a spherical cell grid");
     CR_REQUIRE(cells->nv <= kSeMaxCellV, CR_UNSUPPORTED,
                "spectral-element weights need cells of at most " + std::to_string(kSeMaxCellV) + " vertices");
     CR_REQUIRE(R->fwd.nrows == cells->ncells && R->fwd.ncols == elems->ncells, CR_DIMENSION_MISMATCH,

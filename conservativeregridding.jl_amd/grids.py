@@ -16,6 +16,11 @@ Generators (no GPU needed; they run in libcrhip.so's host code):
 * :func:`ring_full_polygons` -- the cells of a RingGrids full grid
   (ext/ConservativeRegriddingRingGridsExt.jl:22-50)
 * :func:`lonlat_to_unit` -- degree-exact geographic -> unit vectors
+
+Planar grids (the ``Planar`` manifold: ``GO.SpatialTreeInterface.FlatNoTree`` of
+planar polygons, matrices of planar rectangles) are made by
+:meth:`Grid.from_planar_polygons`, :meth:`Grid.from_planar_rings` and
+:meth:`Grid.from_planar_points`; their ``manifold`` is ``Planar()``.
 """
 from __future__ import annotations
 
@@ -193,24 +198,35 @@ def ring_longitudes(nlat_half: int) -> np.ndarray:
     return np.arange(nlon) * (360.0 / nlon)
 
 
-def flatten_tree(tree, share_vertices: bool = True):
+def flatten_tree(tree, share_vertices: bool = True, planar: bool = False):
     """Ragged polygon list ``(vertices, offsets, indices)`` of any object with the
     Trees interface: ``ncells`` (method or attribute) and ``getcell(i)`` (0-based,
     a ring of unit vectors ``(k, 3)`` or lon/lat degrees ``(k, 2)``, open or
-    closed).  With ``share_vertices`` bitwise-equal vertices become one."""
+    closed).  With ``share_vertices`` bitwise-equal vertices become one.
+    ``planar``: the rings are planar ``(k, 2)`` points and stay so."""
     n = tree.ncells() if callable(getattr(tree, "ncells", None)) else int(tree.ncells)
     if n <= 0:
         raise ArgumentError("tree has no cells")
     rings = []
     for i in range(n):
         r = np.asarray(tree.getcell(i), dtype=np.float64)
-        if r.ndim != 2 or r.shape[1] not in (2, 3):
+        if planar:
+            if r.ndim != 2 or r.shape[1] != 2:
+                raise ArgumentError(f"getcell({i}) of a planar tree must return (k, 2) points")
+        elif r.ndim != 2 or r.shape[1] not in (2, 3):
             raise ArgumentError(f"getcell({i}) must return (k, 3) unit vectors or (k, 2) lon/lat degrees")
-        if r.shape[1] == 2:
+        elif r.shape[1] == 2:
             r = lonlat_to_unit(r[:, 0], r[:, 1])
-        if r.shape[0] >= 2 and np.array_equal(r[0], r[-1]):
-            r = r[:-1]
         rings.append(r)
+    return _ragged_from_rings(rings, share_vertices)
+
+
+def _ragged_from_rings(rings, share_vertices: bool):
+    """``(vertices, offsets, indices)`` of a list of rings (open or closed)."""
+    rings = [r[:-1] if r.shape[0] >= 2 and np.array_equal(r[0], r[-1]) else r for r in rings]
+    n = len(rings)
+    if n == 0:
+        raise ArgumentError("no cells")
     lens = np.fromiter((r.shape[0] for r in rings), dtype=np.int64, count=n)
     off = np.zeros(n + 1, dtype=np.int64)
     np.cumsum(lens, out=off[1:])
@@ -253,6 +269,11 @@ class Grid:
         nf = ctypes.c_int64()
         L.check(L.lib.cr_grid_fold_count(self._h, ctypes.byref(nf)))
         self.nfold = nf.value  # (primary, partner) pairs of a folded row
+        pl = ctypes.c_int32()
+        L.check(L.lib.cr_grid_is_planar(self._h, ctypes.byref(pl)))
+        self.planar = bool(pl.value)
+        from .manifolds import Planar, Spherical
+        self.manifold = Planar() if self.planar else Spherical()  # GOCore.best_manifold
 
     # -- constructors ---------------------------------------------------------
     @classmethod
@@ -288,6 +309,54 @@ class Grid:
                                             L.i32ptr(idx), off.size - 1, 1 if convex else 0, _stream(stream)),
                 "Grid.from_polygons")
         return cls(h.value, (off.size - 1,), "polygons")
+
+    @classmethod
+    def from_planar_polygons(cls, vertices, offsets, indices, convex: bool = True, stream=None) -> "Grid":
+        """A grid on the Planar manifold: ragged polygon list of planar points,
+        cell c = ``vertices[indices[offsets[c]:offsets[c+1]]]`` with ``vertices``
+        of shape ``(nverts, 2)`` (3..64 vertices a cell, either orientation; a
+        repeated closing index is dropped).  Regridders between planar grids
+        clip in the plane (the planar operator of regridder.jl:87-94)."""
+        v = np.ascontiguousarray(vertices, dtype=np.float64)
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        idx = np.ascontiguousarray(indices, dtype=np.int32)
+        if v.ndim != 2 or v.shape[1] != 2 or v.shape[0] == 0:
+            raise ArgumentError("planar vertices must have shape (nverts, 2)")
+        if off.ndim != 1 or off.size < 2 or idx.ndim != 1 or off[-1] != idx.size:
+            raise ArgumentError("offsets must be (ncells+1,) with offsets[-1] == len(indices)")
+        h = ctypes.c_void_p()
+        L.check(L.lib.cr_grid_from_planar_polygons(ctypes.byref(h), L.dptr(v), v.shape[0], L.i64ptr(off),
+                                                   L.i32ptr(idx), off.size - 1, 1 if convex else 0,
+                                                   _stream(stream)), "Grid.from_planar_polygons")
+        return cls(h.value, (off.size - 1,), "planar")
+
+    @classmethod
+    def from_planar_rings(cls, rings, convex: bool = True, share_vertices: bool = True, stream=None) -> "Grid":
+        """A planar grid from a list of rings, each ``(k, 2)`` (open or closed):
+        the polygons of a ``FlatNoTree`` (test/usecases/simple.jl:10-26)."""
+        rs = [np.asarray(r, dtype=np.float64) for r in rings]
+        if any(r.ndim != 2 or r.shape[1] != 2 for r in rs):
+            raise ArgumentError("every planar ring must have shape (k, 2)")
+        return cls.from_planar_polygons(*_ragged_from_rings(rs, share_vertices), convex=convex, stream=stream)
+
+    @classmethod
+    def from_planar_points(cls, points, stream=None) -> "Grid":
+        """A planar point matrix ``(ni+1, nj+1, 2)`` (``P[i, j]``): cell
+        ``i + j*ni`` has corners P[i,j], P[i+1,j], P[i+1,j+1], P[i,j+1], the
+        CellBasedGrid order of the spherical point matrices."""
+        p = np.asarray(points, dtype=np.float64)
+        if p.ndim != 3 or p.shape[2] != 2 or p.shape[0] < 2 or p.shape[1] < 2:
+            raise ArgumentError("planar points must have shape (ni+1, nj+1, 2)")
+        W, H = p.shape[0], p.shape[1]
+        ni, nj = W - 1, H - 1
+        verts = np.ascontiguousarray(p.transpose(1, 0, 2).reshape(-1, 2))  # vertex i + j*(ni+1)
+        i, j = np.meshgrid(np.arange(ni), np.arange(nj), indexing="xy")  # (nj, ni): cell i + j*ni
+        v00 = (i + j * W).reshape(-1)
+        idx = np.stack([v00, v00 + 1, v00 + 1 + W, v00 + W], axis=1).astype(np.int32).reshape(-1)
+        off = np.arange(0, 4 * ni * nj + 1, 4, dtype=np.int64)
+        g = cls.from_planar_polygons(verts, off, idx, convex=True, stream=stream)
+        g.shape = (ni, nj)
+        return g
 
     @classmethod
     def from_tree(cls, tree, convex: bool = True, share_vertices: bool = True, stream=None) -> "Grid":
@@ -357,11 +426,18 @@ class Grid:
 
     # -- queries --------------------------------------------------------------
     def cells(self) -> np.ndarray:
+        """The oriented polygon table: ``(ncells, nv, 3)`` unit vectors, or
+        ``(ncells, nv, 2)`` planar points for a planar grid."""
+        if self.planar:
+            out = np.empty((self.ncells, self.nv, 2))
+            L.check(L.lib.cr_grid_copy_planar_cells(self._h, L.dptr(out)), "Grid.cells")
+            return out
         out = np.empty((self.ncells, self.nv, 3))
         L.check(L.lib.cr_grid_copy_cells(self._h, L.dptr(out)), "Grid.cells")
         return out
 
     def unit_areas(self) -> np.ndarray:
+        """Cell areas on the unit sphere (planar areas for a planar grid)."""
         out = np.empty(self.ncells)
         L.check(L.lib.cr_grid_areas(self._h, L.dptr(out)), "Grid.unit_areas")
         return out
@@ -385,6 +461,36 @@ def _stream(stream):
     if isinstance(stream, int):
         return stream
     return getattr(stream, "cuda_stream", None)
+
+
+def as_planar_grid(g) -> Grid:
+    """treeify on the Planar manifold: a planar Grid, a ``(lon_edges, lat_edges)``
+    style tuple of edge vectors (rectangles), a planar point matrix
+    ``(ni+1, nj+1, 2)``, a list of ``(k, 2)`` rings or a tree whose ``getcell``
+    gives ``(k, 2)`` rings."""
+    from .errors import ErrorException, MethodError
+    from .manifolds import _is_edge_tuple
+    if isinstance(g, Grid):
+        if not g.planar:
+            raise ErrorException("Destination and source manifolds must be the same.  Got Spherical() and Planar().")
+        return g
+    if _is_edge_tuple(g):
+        x, y = (np.asarray(v, dtype=np.float64) for v in g)
+        if x.size < 2 or y.size < 2:
+            raise MethodError("a regular grid needs at least two edges in each direction")
+        X, Y = np.meshgrid(x, y, indexing="ij")
+        return Grid.from_planar_points(np.stack([X, Y], axis=-1))
+    if hasattr(g, "getcell") and hasattr(g, "ncells"):
+        verts, off, idx = flatten_tree(g, planar=True)
+        gr = Grid.from_planar_polygons(verts, off, idx)
+        gr.kind = "planar tree"
+        return gr
+    if isinstance(g, (list, tuple)):
+        return Grid.from_planar_rings(g)
+    a = np.asarray(g)
+    if a.dtype.kind in "fi" and a.ndim == 3 and a.shape[-1] == 2:
+        return Grid.from_planar_points(a)
+    raise MethodError(f"cannot treeify {type(g).__name__} on the Planar manifold")
 
 
 def as_grid(g) -> Grid:

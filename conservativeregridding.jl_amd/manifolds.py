@@ -2,12 +2,15 @@
 
 The reference asks each grid for ``GOCore.best_manifold`` and promotes a
 planar grid to the sphere when the other grid is spherical; any other
-mismatch is an error (src/regridder/regridder.jl:105-123).  Grids here hold
-unit vectors, so every device grid is ``Spherical``; the planar input the
-reference's interface knows, a tuple of (lon, lat) edge vectors in degrees
-(``RegularGrid``, src/trees/interfaces.jl:128-129, src/trees/grids.jl:96-115),
-is ``Planar`` and becomes the lon-lat grid of those edges when promoted.
-Planar clipping itself (``FosterHormannClipping``) is not on the device.
+mismatch is an error (src/regridder/regridder.jl:105-123).  A device grid
+knows its manifold (``Grid.manifold``: ``Planar()`` for the planar
+constructors, ``Spherical()`` otherwise); the other planar inputs are a tuple
+of edge vectors (``RegularGrid``, src/trees/interfaces.jl:128-129,
+src/trees/grids.jl:96-115), a planar point matrix ``(ni+1, nj+1, 2)`` and
+objects that say ``manifold = Planar()``.  Promoted to the sphere, planar
+coordinates are read as (lon, lat) degrees.  Two planar grids build on the
+plane: the device clips in the plane where the reference calls
+``FosterHormannClipping(Planar())`` (regridder.jl:87-94).
 """
 from __future__ import annotations
 
@@ -58,12 +61,22 @@ def _is_edge_tuple(g) -> bool:
         return False
 
 
+def _is_planar_points(grid) -> bool:
+    if isinstance(grid, (list, tuple)) or hasattr(grid, "getcell"):
+        return False
+    try:
+        a = np.asarray(grid)
+    except Exception:  # noqa: BLE001 - not array-like
+        return False
+    return a.dtype.kind in "fi" and a.ndim == 3 and a.shape[-1] == 2
+
+
 def best_manifold(grid) -> Manifold:
     """``GOCore.best_manifold(grid)`` for the grid forms the host layer accepts."""
     m = getattr(grid, "manifold", None)
     if isinstance(m, Manifold):
         return m
-    if _is_edge_tuple(grid):
+    if _is_edge_tuple(grid) or _is_planar_points(grid):
         return Planar()
     return Spherical()
 
@@ -85,10 +98,16 @@ def on_sphere(grid):
     """A grid made ready for the spherical build: an edge tuple (lon, lat
     degrees) becomes the point matrix of its lon-lat cells (the RegularGrid's
     cell (i, j) = (x[i], y[j]), (x[i+1], y[j]), (x[i+1], y[j+1]), (x[i], y[j+1]));
+    a planar point matrix or planar device grid is read as lon/lat degrees;
     everything else is returned as it is."""
+    from .grids import Grid, lonlat_to_unit
+    if isinstance(grid, Grid) and grid.planar:
+        return Grid.from_lonlat_cells(grid.cells(), convex=grid.convex)
+    if _is_planar_points(grid):
+        a = np.asarray(grid, dtype=np.float64)
+        return lonlat_to_unit(a[..., 0], a[..., 1])
     if not _is_edge_tuple(grid):
         return grid
-    from .grids import lonlat_to_unit
     x, y = (np.asarray(v, dtype=np.float64) for v in grid)
     if x.size < 2 or y.size < 2:
         raise MethodError("a regular grid needs at least two edges in each direction")
@@ -97,11 +116,10 @@ def on_sphere(grid):
 
 
 def check_buildable(manifold: Manifold) -> float:
-    """The radius of a manifold the device build supports."""
+    """The radius of a manifold the device build supports (1 on the plane:
+    planar areas carry the coordinates' units)."""
     if isinstance(manifold, Spherical):
         return manifold.radius
     if isinstance(manifold, Planar):
-        raise MethodError("Regridder on the Planar manifold: planar clipping (FosterHormannClipping) is not "
-                          "implemented by the device library; give spherical grids, or one spherical grid "
-                          "so that the planar one is promoted")
+        return 1.0
     raise MethodError(f"Regridder on {manifold!r} is not implemented by the device library")

@@ -36,8 +36,8 @@ import numpy as np
 
 from . import _lib as L
 from .errors import ArgumentError, DefaultIntersectionFailureError, DimensionMismatch, MethodError
-from .grids import Grid, as_grid
-from .manifolds import EARTH_RADIUS, Manifold, check_buildable, on_sphere, resolve_manifold
+from .grids import Grid, as_grid, as_planar_grid
+from .manifolds import EARTH_RADIUS, Manifold, Planar, check_buildable, on_sphere, resolve_manifold
 
 
 class _Core:
@@ -104,11 +104,13 @@ class Regridder:
     ``Regridder(dst, src; normalize=false, intersection_operator, threaded)`` of
     src/regridder/regridder.jl:125-163.  ``dst``/``src`` are :class:`Grid` objects,
     unit-vector point matrices ``(ni+1, nj+1, 3)``, objects with the Trees
-    interface, or planar ``(lon_edges, lat_edges)`` tuples in degrees.  The
-    manifold is resolved as ``Regridder(dst, src)`` does (regridder.jl:105-123):
-    a planar grid next to a spherical one is promoted to ``Spherical()``, other
-    mismatches raise ``ErrorException``, and two planar grids raise
-    ``MethodError`` (planar clipping is not on the device).  A pair whose
+    interface, or planar inputs (planar :class:`Grid` objects, ``(x_edges,
+    y_edges)`` tuples, ``(ni+1, nj+1, 2)`` point matrices).  The manifold is
+    resolved as ``Regridder(dst, src)`` does (regridder.jl:105-123): a planar
+    grid next to a spherical one is promoted to ``Spherical()`` (its
+    coordinates read as lon/lat degrees), other mismatches raise
+    ``ErrorException``, and two planar grids build on the plane (clip in the
+    plane, shoelace areas, no radius: regridder.jl:87-94).  A pair whose
     intersection area is not finite raises ``DefaultIntersectionFailureError``
     with the two cells.  Extra keywords:
 
@@ -149,7 +151,10 @@ class Regridder:
             h, self.dst_shape, self.src_shape = build_se_regridder(dst, src, opts, triangle_quad_degree, stream)
             self._core, self._t = _Core(h), False
             return
-        dg, sg = as_grid(on_sphere(dst)), as_grid(on_sphere(src))
+        if isinstance(manifold, Planar):
+            dg, sg = as_planar_grid(dst), as_planar_grid(src)
+        else:
+            dg, sg = as_grid(on_sphere(dst)), as_grid(on_sphere(src))
         h = ctypes.c_void_p()
         st = L.lib.cr_regridder_build(ctypes.byref(h), dg._h, sg._h, ctypes.byref(opts), _stream(stream))
         if st == L.CR_DEGENERATE_GEOMETRY:

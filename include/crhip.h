@@ -174,6 +174,27 @@ cr_status cr_grid_from_points(cr_grid **grid, const double *pts, int64_t nblocks
 cr_status cr_grid_from_polygons(cr_grid **grid, const double *xyz, int64_t nverts,
                                 const int64_t *offsets, const int32_t *verts, int64_t ncells,
                                 int32_t convex, void *stream);
+/* A grid on the Planar manifold: the ragged polygon list of cr_grid_from_polygons
+ * with 2-D vertices (x, y interleaved, nverts of them), for
+ * Regridder(GO.Planar(), dst, src) -- the planar DefaultIntersectionOperator
+ * (src/regridder/regridder.jl:87-94) on grids given as polygons
+ * (GO.SpatialTreeInterface.FlatNoTree(polys) of test/usecases/simple.jl, the
+ * planar rectangle matrices of test/regridding.jl:299-376).  cr_regridder_build
+ * of two planar grids clips the rings in the plane and measures areas with the
+ * shoelace formula (planar units, no radius); a planar grid next to a
+ * spherical one fails with CR_INVALID_ARGUMENT ("manifolds must be the same").
+ * Rings of either orientation are turned counter-clockwise; `convex` declares
+ * every cell convex (one side of a build must be).  The vertices must span a
+ * non-empty box and be finite (CR_INVALID_ARGUMENT).  Vertex gradients,
+ * cell averages of the example fields and spectral-element weights are
+ * spherical and return CR_UNSUPPORTED on planar grids. */
+cr_status cr_grid_from_planar_polygons(cr_grid **grid, const double *xy, int64_t nverts,
+                                       const int64_t *offsets, const int32_t *verts, int64_t ncells,
+                                       int32_t convex, void *stream);
+/* 1 if the grid is on the Planar manifold (GOCore.best_manifold), else 0. */
+cr_status cr_grid_is_planar(const cr_grid *grid, int32_t *planar);
+/* The oriented planar polygon table of a planar grid: ncells*nv*2 doubles. */
+cr_status cr_grid_copy_planar_cells(const cr_grid *grid, double *cells_xy);
 /* Tripolar grid with a RightCenterFolded top row from its (nx+1) x (ny+1)
  * point matrix (one block, layout of cr_grid_from_points): treeify of
  * ZRegOrthogonalSphericalShellGrid{..., RightCenterFolded}
@@ -193,9 +214,11 @@ cr_status cr_grid_info(const cr_grid *grid, int64_t *ncells, int32_t *nv, int32_
  * block), ncells*nv for a polygon table or a HEALPix grid.  Vertex gradients
  * (cr_regrid_backward) are indexed by these. */
 cr_status cr_grid_vertex_count(const cr_grid *grid, int64_t *nverts);
-/* Download the (re-oriented) polygon table, ncells*nv*3 doubles. */
+/* Download the (re-oriented) polygon table, ncells*nv*3 doubles (for a planar
+ * grid: the unit-vector images its candidate search uses). */
 cr_status cr_grid_copy_cells(const cr_grid *grid, double *cells);
-/* Cell areas on the unit sphere, computed on the device (areas(), regridder.jl:165-178). */
+/* Cell areas on the unit sphere, or in the plane for a planar grid, computed on
+ * the device (areas(), regridder.jl:165-178). */
 cr_status cr_grid_areas(const cr_grid *grid, double *areas_host);
 void cr_grid_destroy(cr_grid *grid);
 

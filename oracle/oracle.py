@@ -11,6 +11,8 @@ for the file:line map):
 * :func:`transpose`  -- the CSR of ``transpose(R).intersections`` (regridder.jl:49-50)
 * :func:`clip_area`, :func:`polygon_area`, :func:`cell_areas` -- the per-pair
   operator (regridder.jl:96-103) and ``areas`` (regridder.jl:165-178)
+* :func:`build_planar`, :func:`planar_clip_area`, :func:`planar_polygon_area`
+  -- the same on the Planar manifold (regridder.jl:87-94; cr_oracle_planar.c)
 * :func:`tile_plan`  -- the library's own staged-apply layout (cr_tileplan.cpp;
   no reference counterpart), the checker of the device planner
 * :func:`dunavant`, :func:`gll`, :func:`lagrange`, :func:`element_map`,
@@ -105,6 +107,22 @@ def lib():
     L.oracle_tile_plan_copy.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 7
     L.oracle_tile_plan_free.restype = None
     L.oracle_tile_plan_free.argtypes = [ctypes.c_void_p]
+    L.orc_planar_polygon_area.restype = ctypes.c_double
+    L.orc_planar_polygon_area.argtypes = [_dp, ctypes.c_int32]
+    L.orc_planar_clip_area.restype = ctypes.c_double
+    L.orc_planar_clip_area.argtypes = [_dp, ctypes.c_int32, _dp, ctypes.c_int32]
+    L.orc_planar_build.restype = ctypes.c_void_p
+    L.orc_planar_build.argtypes = [_dp, ctypes.c_int64, ctypes.c_int32, _dp, ctypes.c_int64, ctypes.c_int32,
+                                   ctypes.c_int32, ctypes.c_double]
+    for name in ("orc_planar_nnz", "orc_planar_candidates"):
+        getattr(L, name).restype = ctypes.c_int64
+        getattr(L, name).argtypes = [ctypes.c_void_p]
+    L.orc_planar_status.restype = ctypes.c_int32
+    L.orc_planar_status.argtypes = [ctypes.c_void_p]
+    L.orc_planar_copy.restype = None
+    L.orc_planar_copy.argtypes = [ctypes.c_void_p, _i64p, _i32p, _dp, _dp, _dp]
+    L.orc_planar_free.restype = None
+    L.orc_planar_free.argtypes = [ctypes.c_void_p]
     _lib = L
     return L
 
@@ -304,6 +322,44 @@ def build(dst_cells, src_cells, clip_is_dst: bool = True, threshold_rel: float =
         L.orc_result_free(h)
     return OracleCSR(d.shape[0], s.shape[0], rowptr, col, val, a_dst, a_src, ncand, float(t[0]), nth, flops,
                      flops_kept)
+
+
+def planar_polygon_area(ring) -> float:
+    """Shoelace area of an open ring ``(k, 2)`` (signed, counter-clockwise > 0)."""
+    r = _f64(ring)
+    return lib().orc_planar_polygon_area(_d(r), r.shape[0])
+
+
+def planar_clip_area(subject, clip) -> float:
+    """Planar area of ``subject ∩ clip`` (open rings ``(k, 2)``, clip convex and
+    counter-clockwise): the planar operator of regridder.jl:87-94."""
+    s, c = _f64(subject), _f64(clip)
+    return lib().orc_planar_clip_area(_d(s), s.shape[0], _d(c), c.shape[0])
+
+
+def build_planar(dst_cells, src_cells, clip_is_dst: bool = True, threshold_rel: float = 1e-14) -> OracleCSR:
+    """Regridder build on the Planar manifold from planar polygon tables
+    ``(ncells, nv, 2)`` (open counter-clockwise rings)."""
+    d, s = _f64(dst_cells), _f64(src_cells)
+    L = lib()
+    h = L.orc_planar_build(_d(d), d.shape[0], d.shape[1], _d(s), s.shape[0], s.shape[1],
+                           1 if clip_is_dst else 0, threshold_rel)
+    try:
+        st = L.orc_planar_status(h)
+        if st != 0:
+            raise RuntimeError(f"oracle planar build failed (status {st})")
+        nnz = L.orc_planar_nnz(h)
+        rowptr = np.empty(d.shape[0] + 1, np.int64)
+        col = np.empty(nnz, np.int32)
+        val = np.empty(nnz, np.float64)
+        a_dst = np.empty(d.shape[0])
+        a_src = np.empty(s.shape[0])
+        L.orc_planar_copy(h, rowptr.ctypes.data_as(_i64p), col.ctypes.data_as(_i32p), _d(val), _d(a_dst),
+                          _d(a_src))
+        ncand = L.orc_planar_candidates(h)
+    finally:
+        L.orc_planar_free(h)
+    return OracleCSR(d.shape[0], s.shape[0], rowptr, col, val, a_dst, a_src, ncand, 0.0, 1)
 
 
 def spmv(rowptr, col, val, x, area=None, nthreads: int = 1) -> np.ndarray:

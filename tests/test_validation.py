@@ -136,9 +136,15 @@ def test_manifold_resolution(cr):
         cr.resolve_manifold(OnEllipsoid(), pts)
     with pytest.raises(cr.ErrorException):
         cr.Regridder(pts, OnEllipsoid())
-    # two planar grids: planar clipping is not on the device (raised before any device call)
-    with pytest.raises(cr.MethodError, match="Planar"):
-        cr.Regridder(edges, edges)
+    # two planar grids build on the plane: the planar grid constructor is
+    # reached (without a GPU it is the first call that needs the device)
+    planar_pts = np.stack(np.meshgrid(edges[0], edges[1], indexing="ij"), axis=-1)
+    assert cr.best_manifold(planar_pts) == Planar()
+    assert cr.resolve_manifold(planar_pts, edges) == Planar()
+    assert cr.resolve_manifold(planar_pts, pts) == Spherical()
+    if cr.device_count() == 0:
+        with pytest.raises(cr.CRError, match="from_planar_polygons"):
+            cr.Regridder(edges, edges)
 
 
 def test_promoted_planar_grid_is_the_lonlat_grid_of_its_edges(cr):
