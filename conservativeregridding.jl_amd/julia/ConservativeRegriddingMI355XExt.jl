@@ -7,7 +7,7 @@
 #
 #   * Regridder(manifold, dst, src) for device grids -> cr_regridder_build
 #     (src/regridder/regridder.jl:125-163: treeify, intersection_areas, areas,
-#     normalize!);
+#     normalize!), on the Spherical or the Planar manifold;
 #   * regrid!(dst, R, src; dims, normalize) for ROCArray fields of any rank:
 #     extract_source_arraylike / extract_dest_arraylike return a DeviceSlicer (an
 #     AbstractDimensionalSlicer, src/regridder/regrid.jl:157-318) that carries
@@ -147,6 +147,41 @@ function DeviceGrid(manifold::GO.Spherical, grid; convex::Bool = true)
     return DeviceGrid(h[], ncell)
 end
 
+# The same on the Planar manifold: the polygons of any planar tree or polygon
+# collection (GO.SpatialTreeInterface.FlatNoTree(polys), test/usecases/simple.jl;
+# matrices of planar rectangles, test/regridding.jl:299-376) as 2-D points
+# (cr_grid_from_planar_polygons): regridders between two such grids clip in the
+# plane, the DefaultIntersectionOperator{Planar} of regridder.jl:87-94.
+function DeviceGrid(manifold::GO.Planar, grid; convex::Bool = true)
+    tree = Trees.treeify(manifold, grid)
+    id = Dict{NTuple{2,Float64},Int32}()
+    xy = Float64[]
+    offsets = Int64[0]
+    verts = Int32[]
+    for cell in Trees.getcell(tree)
+        ring = collect(GI.getpoint(GI.getexterior(cell)))
+        n = length(ring)
+        if n > 1 && (GI.x(ring[1]), GI.y(ring[1])) == (GI.x(ring[end]), GI.y(ring[end]))
+            n -= 1
+        end
+        for p in view(ring, 1:n)
+            key = (Float64(GI.x(p)), Float64(GI.y(p)))
+            k = get!(id, key) do
+                append!(xy, key)
+                Int32(length(xy) ÷ 2 - 1)
+            end
+            push!(verts, k)
+        end
+        push!(offsets, length(verts))
+    end
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    ncell = length(offsets) - 1
+    check(ccall((:cr_grid_from_planar_polygons, libcrhip), Cint,
+                (Ref{Ptr{Cvoid}}, Ptr{Float64}, Int64, Ptr{Int64}, Ptr{Int32}, Int64, Int32, Ptr{Cvoid}),
+                h, xy, length(xy) ÷ 2, offsets, verts, ncell, convex, C_NULL))
+    return DeviceGrid(h[], ncell)
+end
+
 # HEALPix (HealpixRootNode{O}(nside), ext/ConservativeRegriddingHealpixExt.jl):
 # nested or ring order, `step` boundary points per side (1: the reference's 4-gons).
 function healpix_grid(nside::Integer; nested::Bool = true, step::Integer = 1)
@@ -221,6 +256,22 @@ function Regridder(manifold::GO.Spherical, dst::DeviceGrid, src::DeviceGrid;
     intersection_operator === nothing ||
         throw(ArgumentError("a per-pair intersection_operator cannot run inside the device build"))
     opts = Ref(BuildOpts(threshold_rel, manifold.radius, normalize, 0, 0, 0))
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:cr_regridder_build, libcrhip), Cint,
+                (Ref{Ptr{Cvoid}}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{BuildOpts}, Ptr{Cvoid}),
+                h, dst.h, src.h, opts, C_NULL))
+    return wrap_regridder(DeviceHandle(h[]), dst.ncells, src.ncells)
+end
+
+# Regridder(GO.Planar(), dst, src; ...) for two planar device grids: planar
+# areas, no radius (a spherical grid on either side fails in cr_regridder_build
+# with the reference's "manifolds must be the same").
+function Regridder(::GO.Planar, dst::DeviceGrid, src::DeviceGrid;
+                   normalize::Bool = false, threshold_rel::Real = 1e-14, intersection_operator = nothing,
+                   kwargs...)
+    intersection_operator === nothing ||
+        throw(ArgumentError("a per-pair intersection_operator cannot run inside the device build"))
+    opts = Ref(BuildOpts(threshold_rel, 1.0, normalize, 0, 0, 0))
     h = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:cr_regridder_build, libcrhip), Cint,
                 (Ref{Ptr{Cvoid}}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{BuildOpts}, Ptr{Cvoid}),
