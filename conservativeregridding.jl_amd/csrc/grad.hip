@@ -56,6 +56,7 @@ struct VertexSink {
     double *lacc_subj, *lacc_clip;     // LDS accumulators of one side's slots, or null
     double *own_clip;                  // the pair's own 4 x 3 clip-slot sums in LDS, or null
     double *own_subj;                  // the pair's own 4 x 3 subject-slot sums in LDS, or null
+    bool own_atomic = false;           // other lanes add to the own sums too (LDS atomics)
 
     // slot e of the subject, 0 <= e <= ns (an edge's end vertex e + 1 wraps)
     __device__ int wrap(int e) const { return e >= ns ? e - ns : e; }
@@ -79,11 +80,15 @@ struct VertexSink {
         e = wrap(e);
         if (lacc_subj) {
             add_local(lacc_subj, e, tangential(g, sv(e)));
-        } else if (own_subj) {  // this pair's own sums (no other thread touches them)
+        } else if (own_subj) {  // this pair's own sums
             const d3 t = tangential(g, sv(e));
-            own_subj[3 * e] += t.x;
-            own_subj[3 * e + 1] += t.y;
-            own_subj[3 * e + 2] += t.z;
+            if (own_atomic) {
+                add_local(own_subj, e, t);
+            } else {
+                own_subj[3 * e] += t.x;
+                own_subj[3 * e + 1] += t.y;
+                own_subj[3 * e + 2] += t.z;
+            }
         } else {
             add(g_subj, subj_vi[e], tangential(g, sv(e)));
         }
@@ -94,10 +99,14 @@ struct VertexSink {
         const d3 t = tangential(g, cv(kk));
         if (lacc_clip) {
             add_local(lacc_clip, kk, t);
-        } else if (own_clip) {  // this pair's own sums (no other thread touches them)
-            own_clip[3 * kk] += t.x;
-            own_clip[3 * kk + 1] += t.y;
-            own_clip[3 * kk + 2] += t.z;
+        } else if (own_clip) {  // this pair's own sums
+            if (own_atomic) {
+                add_local(own_clip, kk, t);
+            } else {
+                own_clip[3 * kk] += t.x;
+                own_clip[3 * kk + 1] += t.y;
+                own_clip[3 * kk + 2] += t.z;
+            }
         } else {
             add(g_clip, clip_vi[kk], t);
         }
@@ -130,58 +139,38 @@ struct VertexSink {
 // pair; other contributions use global FP64 atomics directly.
 constexpr int kGradRows = 16;
 
-// Two waves per SIMD (256 registers): the second wave hides the atomics' and
-// the FP64 chain's latency.  The clip pipeline's edge normals live in LDS (6 KB
-// per workgroup), which cuts the register spills from 73 to 24; moving its
-// first vertices and clip vertices there too removes the rest but lowers the
-// occupancy LDS allows and runs slower (C3 vertex pass: 8.4 ms with the
-// normals in registers, 7.1 ms as here, 9.1 ms with none spilled).
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_clip_grad(
-    const int32_t *__restrict__ rowid, const int32_t *__restrict__ col, const double *__restrict__ coef,
-    int64_t nnz, const double *__restrict__ dst_cells, int dst_nv, const int32_t *__restrict__ dst_vi,
-    const double *__restrict__ src_cells, int src_nv, const int32_t *__restrict__ src_vi,
-    int clip_is_dst, double *__restrict__ g_dst, double *__restrict__ g_src) {
-    extern __shared__ double lacc[];  // kGradRows * dst_nv * 3
-    __shared__ double own[64 * 12];
-    __shared__ d3 nrm_lds[64 * 4];  // each thread's 4 clip-edge normals
-    const int64_t k0 = (int64_t)blockIdx.x * 64, k = k0 + threadIdx.x;
-    const int64_t r0 = rowid[k0];
-    const int nacc = kGradRows * dst_nv * 3;
-    for (int i = threadIdx.x; i < nacc; i += 64) lacc[i] = 0.0;
-    double *my_own = own + 12 * threadIdx.x;
-#pragma unroll
-    for (int i = 0; i < 12; ++i) my_own[i] = 0.0;
-    __syncthreads();
-    if (k < nnz && coef[k] != 0.0) {
-        const double c = coef[k];
-        const int64_t d = rowid[k], s = col[k];
-        const double *dp = dst_cells + 3 * (int64_t)dst_nv * d;
-        const double *sp = src_cells + 3 * (int64_t)src_nv * s;
-        double *mine = (g_dst && d - r0 < kGradRows) ? lacc + (d - r0) * dst_nv * 3 : nullptr;
-        VertexSink sink;
-        if (clip_is_dst) {
-            // a source subject of at most 4 vertices sums its slots per pair
-            sink = VertexSink{sp, dp, src_nv, dst_nv, src_vi + (int64_t)src_nv * s, dst_vi + (int64_t)dst_nv * d,
-                              g_src, g_dst, nullptr, mine, nullptr, (g_src && src_nv <= 4) ? my_own : nullptr};
-        } else {
-            sink = VertexSink{dp, sp, dst_nv, src_nv, dst_vi + (int64_t)dst_nv * d, src_vi + (int64_t)src_nv * s,
-                              g_dst, g_src, mine, nullptr, g_src ? my_own : nullptr, nullptr};
-        }
-        clip_area_backward<4, VertexSink>(sink.clip, sink.nc, sink.subj, sink.ns, c, sink,
-                                          nrm_lds + 4 * threadIdx.x);
-        // the per-pair sums of the source cell's 4 slots, one atomic per component
-        if (sink.own_clip || sink.own_subj) {
-            const int32_t *vi = sink.own_clip ? sink.clip_vi : sink.subj_vi;
-            for (int kk = 0; kk < src_nv && kk < 4; ++kk) {
-                const d3 v = mk3(my_own[3 * kk], my_own[3 * kk + 1], my_own[3 * kk + 2]);
-                if (v.x != 0.0 || v.y != 0.0 || v.z != 0.0) VertexSink::add(g_src, vi[kk], v);
-            }
-        }
+// The sink of one pair: its two rings, its slots' accumulators (the wave's
+// destination rows in LDS when `lacc` is given and the row is one of them,
+// the pair's own source-cell sums in `own`) and the caller-vertex gradients.
+__device__ __forceinline__ VertexSink pair_sink(int64_t d, int64_t s, int64_t r0, double *lacc, double *own,
+                                                const double *__restrict__ dst_cells, int dst_nv,
+                                                const int32_t *__restrict__ dst_vi,
+                                                const double *__restrict__ src_cells, int src_nv,
+                                                const int32_t *__restrict__ src_vi, int clip_is_dst,
+                                                double *__restrict__ g_dst, double *__restrict__ g_src) {
+    const double *dp = dst_cells + 3 * (int64_t)dst_nv * d;
+    const double *sp = src_cells + 3 * (int64_t)src_nv * s;
+    double *mine = (lacc && g_dst && d - r0 < kGradRows) ? lacc + (d - r0) * dst_nv * 3 : nullptr;
+    if (clip_is_dst)  // a source subject of at most 4 vertices sums its slots per pair
+        return VertexSink{sp, dp, src_nv, dst_nv, src_vi + (int64_t)src_nv * s, dst_vi + (int64_t)dst_nv * d,
+                          g_src, g_dst, nullptr, mine, nullptr, (g_src && src_nv <= 4) ? own : nullptr};
+    return VertexSink{dp, sp, dst_nv, src_nv, dst_vi + (int64_t)dst_nv * d, src_vi + (int64_t)src_nv * s,
+                      g_dst, g_src, mine, nullptr, g_src ? own : nullptr, nullptr};
+}
+
+// The per-pair sums of the source cell's 4 slots, one atomic per component.
+__device__ __forceinline__ void flush_own(const VertexSink &sink, int src_nv, double *own, double *g_src) {
+    if (!sink.own_clip && !sink.own_subj) return;
+    const int32_t *vi = sink.own_clip ? sink.clip_vi : sink.subj_vi;
+    for (int kk = 0; kk < src_nv && kk < 4; ++kk) {
+        const d3 v = mk3(own[3 * kk], own[3 * kk + 1], own[3 * kk + 2]);
+        if (v.x != 0.0 || v.y != 0.0 || v.z != 0.0) VertexSink::add(g_src, vi[kk], v);
     }
-    __syncthreads();
-    if (!g_dst) return;
-    const int64_t klast = min(nnz, k0 + 64) - 1;
-    const int64_t nrows = min((int64_t)kGradRows, (int64_t)rowid[klast] - r0 + 1);
+}
+
+// The wave's destination rows from LDS to memory.
+__device__ __forceinline__ void flush_rows(const double *lacc, int64_t r0, int64_t nrows, int dst_nv,
+                                           const int32_t *__restrict__ dst_vi, double *g_dst) {
     for (int i = threadIdx.x; i < nrows * dst_nv * 3; i += 64) {
         const double v = lacc[i];
         if (v == 0.0) continue;
@@ -189,6 +178,186 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
         atomicAdd(g_dst + 3 * (int64_t)dst_vi[(r0 + r) * dst_nv + slot] + comp, v);
     }
 }
+
+// Crossing work of the boundary decomposition, queued per wave in LDS: a
+// crossing's gradient (tangential at it) with its label and its pair's lane,
+// done 64 at a time by the whole wave at the points where every lane's pass 2
+// is in step.  A flush leaves fewer than 64; a lane adds at most 2 between
+// flushes.
+constexpr int kXQueue = 64 + 2 * 64;
+
+struct XQueue {
+    int32_t lab[kXQueue], lane[kXQueue];
+    double g[3 * kXQueue];
+    int32_t pd[64], ps[64];  // each lane's pair (destination, source cell)
+    int n;
+};
+
+struct QueueEmit {
+    VertexSink *sink;
+    XQueue *q;
+    bool live;
+    // the decomposition emits subject vertices, clip vertices and
+    // subject x clip crossings (no clip x clip crossings)
+    __device__ __forceinline__ void operator()(int32_t l, const d3 &x, const d3 &g) {
+        if (!live) return;
+        const int32_t t = lab_type(l);
+        if (t == kLabXF) {
+            const d3 gt = tangential(g, x);
+            const int e = atomicAdd(&q->n, 1);
+            q->lab[e] = l;
+            q->lane[e] = threadIdx.x;
+            q->g[3 * e] = gt.x;
+            q->g[3 * e + 1] = gt.y;
+            q->g[3 * e + 2] = gt.z;
+        } else if (t == kLabS) {
+            sink->add_subj(l & kLabIndexMask, g);
+        } else {
+            sink->add_clip(lab_lo(l), g);
+        }
+    }
+};
+
+// The vertex pass by boundary decomposition (edges_plan + edges_emit), one
+// wave per 64 consecutive non-zeros: flat loops over the subject edges and
+// the clip edges (no clip stages), every lane through the same iterations
+// (a declined or empty lane emits nothing), crossings through the queue.  A
+// pair the decomposition declines (near a degenerate configuration, or a
+// degenerate clip edge) is appended to `list` (wave by wave, in order) for
+// k_clip_grad_list.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_clip_grad_edges(
+    const int32_t *__restrict__ rowid, const int32_t *__restrict__ col, const double *__restrict__ coef,
+    int64_t nnz, const double *__restrict__ dst_cells, int dst_nv, const int32_t *__restrict__ dst_vi,
+    const double *__restrict__ src_cells, int src_nv, const int32_t *__restrict__ src_vi, int clip_is_dst,
+    double *__restrict__ g_dst, double *__restrict__ g_src, int32_t *__restrict__ list,
+    uint32_t *__restrict__ count) {
+    extern __shared__ double lacc[];  // kGradRows * dst_nv * 3
+    __shared__ double own[64 * 12];
+    __shared__ XQueue q;
+    const int lane = threadIdx.x;
+    const int64_t k0 = (int64_t)blockIdx.x * 64, k = k0 + lane;
+    const int64_t r0 = rowid[k0];
+    const int nacc = kGradRows * dst_nv * 3;
+    for (int i = lane; i < nacc; i += 64) lacc[i] = 0.0;
+    double *my_own = own + 12 * lane;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) my_own[i] = 0.0;
+    const bool real = k < nnz;
+    const int64_t kk = real ? k : k0;  // lanes past the end run the loops on a valid pair
+    const int32_t d = rowid[kk], s = col[kk];
+    q.pd[lane] = d;
+    q.ps[lane] = s;
+    if (lane == 0) q.n = 0;
+    __syncthreads();
+    const double c = real ? coef[kk] : 0.0;
+    VertexSink sink = pair_sink(d, s, r0, lacc, my_own, dst_cells, dst_nv, dst_vi, src_cells, src_nv, src_vi,
+                                clip_is_dst, g_dst, g_src);
+    EdgesPlan<4> P;
+    const bool planned = edges_plan<4>(sink.clip, sink.nc, sink.subj, sink.ns, P);
+    const bool live = c != 0.0 && planned;
+    // declined pairs to the list: one atomic per wave, lanes in order
+    {
+        const bool fb = c != 0.0 && !planned;
+        const uint64_t m = __ballot(fb);
+        if (m) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(count, (uint32_t)__popcll(m));
+            base = __shfl(base, 0);
+            if (fb) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)k;
+        }
+    }
+    auto work = [&](int e) {
+        const int L = q.lane[e];
+        const int32_t l = q.lab[e];
+        const d3 gt = mk3(q.g[3 * e], q.g[3 * e + 1], q.g[3 * e + 2]);
+        VertexSink sk = pair_sink(q.pd[L], q.ps[L], r0, lacc, own + 12 * L, dst_cells, dst_nv, dst_vi, src_cells,
+                                  src_nv, src_vi, clip_is_dst, g_dst, g_src);
+        sk.own_atomic = true;
+        const int a = lab_hi(l), b = lab_lo(l);
+        d3 gp1, gq1, gp2, gq2;
+        if (crossing_backward_arc(gt, sk.sv(a), sk.sv(a + 1), sk.cv(b), sk.cv(b + 1), gp1, gq1, gp2, gq2)) {
+            sk.add_subj(a, gp1);
+            sk.add_subj(a + 1, gq1);
+            sk.add_clip(b, gp2);
+            sk.add_clip(b + 1, gq2);
+        }
+    };
+    auto flush = [&](bool drain) {
+        __syncthreads();
+        int n = q.n;
+        while (n >= 64) {
+            work(n - 64 + lane);
+            n -= 64;
+        }
+        if (drain && lane < n) work(lane);
+        __syncthreads();
+        if (lane == 0) q.n = drain ? 0 : n;
+        __syncthreads();
+    };
+    QueueEmit em{&sink, &q, live};
+    edges_emit<4>(P, sink.subj, sink.ns, c, em, [&] { flush(false); });
+    flush(true);
+    if (live) flush_own(sink, src_nv, my_own, g_src);
+    __syncthreads();
+    if (!g_dst) return;
+    const int64_t klast = min(nnz, k0 + 64) - 1;
+    flush_rows(lacc, r0, min((int64_t)kGradRows, (int64_t)rowid[klast] - r0 + 1), dst_nv, dst_vi, g_dst);
+}
+
+// The vertex pass through the clip pipeline (clip_area_backward) for the
+// pairs the boundary decomposition declines: list[0 .. *count), 64 per wave,
+// the waves striding over the list.  The list holds each wave's declined
+// pairs in order, so a window of 64 entries mostly shares rows: rows r0 ..
+// r0 + kGradRows - 1 of its first entry are summed in LDS, the rest go to
+// memory directly.  Two waves per SIMD (256 registers): the second wave hides
+// the atomics' and the FP64 chain's latency.  The clip pipeline's edge
+// normals live in LDS (6 KB per workgroup), which cuts the register spills
+// from 73 to about 30; moving its first vertices and clip vertices there too
+// removes the rest but lowers the occupancy LDS allows and ran slower (as the
+// whole vertex pass on C3: 8.4 ms with the normals in registers, 7.1 ms as
+// here, 9.1 ms with none spilled), and so did one wave per SIMD with no
+// spills (8.3 ms against 5.4).
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_clip_grad_list(
+    const int32_t *__restrict__ rowid, const int32_t *__restrict__ col, const double *__restrict__ coef,
+    const double *__restrict__ dst_cells, int dst_nv, const int32_t *__restrict__ dst_vi,
+    const double *__restrict__ src_cells, int src_nv, const int32_t *__restrict__ src_vi, int clip_is_dst,
+    double *__restrict__ g_dst, double *__restrict__ g_src, const int32_t *__restrict__ list,
+    const uint32_t *__restrict__ count, int64_t n_rows) {
+    extern __shared__ double lacc[];  // kGradRows * dst_nv * 3
+    __shared__ double own[64 * 12];
+    __shared__ d3 nrm_lds[64 * 4];  // each thread's 4 clip-edge normals
+    double *my_own = own + 12 * threadIdx.x;
+    const int64_t total = *count;
+    const int nacc = kGradRows * dst_nv * 3;
+    for (int64_t base = (int64_t)blockIdx.x * 64; base < total; base += (int64_t)gridDim.x * 64) {
+        const int64_t r0 = rowid[list[base]];
+        for (int i = threadIdx.x; i < nacc; i += 64) lacc[i] = 0.0;
+#pragma unroll
+        for (int i = 0; i < 12; ++i) my_own[i] = 0.0;
+        __syncthreads();
+        const int64_t i = base + threadIdx.x;
+        if (i < total) {
+            const int64_t k = list[i];
+            const double c = coef[k];
+            const int64_t d = rowid[k];
+            if (c != 0.0) {
+                VertexSink sink = pair_sink(d, col[k], r0, d >= r0 ? lacc : nullptr, my_own, dst_cells, dst_nv,
+                                            dst_vi, src_cells, src_nv, src_vi, clip_is_dst, g_dst, g_src);
+                clip_area_backward<4, VertexSink>(sink.clip, sink.nc, sink.subj, sink.ns, c, sink,
+                                                  nrm_lds + 4 * threadIdx.x);
+                flush_own(sink, src_nv, my_own, g_src);
+            }
+        }
+        __syncthreads();
+        if (g_dst) flush_rows(lacc, r0, min((int64_t)kGradRows, n_rows - r0), dst_nv, dst_vi, g_dst);
+        __syncthreads();
+    }
+}
+
+__global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
+
+// Workgroups of the list pass (the waves stride over the declined pairs).
+constexpr unsigned kListBlocks = 2048;
 
 // d(area of each destination cell)/dV times dL/da_u.  private_v: no vertex
 // index is shared by two cells, so each thread owns its cell's entries of g
@@ -265,6 +434,49 @@ __global__ void __launch_bounds__(64) k_area_grad_private(const double *__restri
     for (int i = threadIdx.x; i < n; i += 64) gb[i] = gout[i];
 }
 
+// The same for tables of NV-vertex cells whose slot -> vertex map is the
+// identity (cells own their vertices and none was re-oriented): a lane keeps
+// its cell's NV slot gradients in registers (the fan's slots are compile-time
+// indices), and one LDS buffer stages the block's vertices in and the
+// gradients out, both with coalesced accesses.
+template <int NV>
+__global__ void __launch_bounds__(64) k_area_grad_table(const double *__restrict__ cells, int64_t ncells,
+                                                        double kappa, const double *__restrict__ a_to, int64_t nf,
+                                                        const double *__restrict__ y, int64_t yc, int64_t yf,
+                                                        const double *__restrict__ gy, int64_t gc, int64_t gf,
+                                                        double *__restrict__ g) {
+    __shared__ double stage[64 * NV * 3];
+    const int64_t c0 = (int64_t)blockIdx.x * 64;
+    const int ncell = (int)min((int64_t)64, ncells - c0);
+    const int n = ncell * NV * 3;
+    const double *src = cells + 3 * (int64_t)NV * c0;
+    double *gb = g + 3 * (int64_t)NV * c0;
+    for (int i = threadIdx.x; i < n; i += 64) stage[i] = src[i];
+    __syncthreads();
+    const int t = threadIdx.x;
+    const int64_t c = c0 + t;
+    d3 gr[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) gr[k] = mk3(0.0, 0.0, 0.0);
+    if (t < ncell && a_to[c] != 0.0) {  // zero-area (ghost) cells have no area term
+        double sum = 0.0;
+        for (int64_t f = 0; f < nf; ++f) sum += gy[c * gc + f * gf] * y[c * yc + f * yf];
+        const double coef = -kappa * sum / a_to[c];
+        if (coef != 0.0) ring_area_backward_n<NV>(stage + 3 * NV * t, coef, gr);
+    }
+    __syncthreads();
+    if (t < ncell) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            stage[3 * (NV * t + k)] = gr[k].x;
+            stage[3 * (NV * t + k) + 1] = gr[k].y;
+            stage[3 * (NV * t + k) + 2] = gr[k].z;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 64) gb[i] += stage[i];
+}
+
 __global__ void k_div_rows(const double *__restrict__ gy, int64_t gc, int64_t gf, int64_t n, int64_t nf,
                            const double *__restrict__ a, double *__restrict__ w) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -279,10 +491,13 @@ void apply(const cr_regridder *R, bool transpose, int dtype, int64_t nf, const v
 bool apply_ready(const cr_regridder *R, bool transpose, int dtype, int64_t nf, int64_t sf, int64_t df);
 
 // Scratch of the reverse mode: w (n_to x nfields doubles, the field adjoint's
-// input), coef (a double per non-zero) and rowid (an int32 per non-zero).
+// input), coef (a double per non-zero), rowid (an int32 per non-zero), list
+// (the non-zeros the boundary decomposition declines, at most one int32 per
+// non-zero) and its count.
 struct BackwardScratch {
     double *w = nullptr, *coef = nullptr;
-    int32_t *rowid = nullptr;
+    int32_t *rowid = nullptr, *list = nullptr;
+    uint32_t *count = nullptr;
 };
 
 // The reverse mode on the given stream.  ws == nullptr: the scratch is
@@ -350,15 +565,44 @@ static void backward(const cr_regridder *R, const cr_grid *dst, const cr_grid *s
         hipLaunchKernelGGL(k_pair_coef, dim3(grid_for(F.nnz, B)), dim3(B), 0, s, rowid, F.col.p, F.nnz,
                            (int)(transpose != 0), (int)(normalize != 0), kappa, a_to, nfields, x, xc, xf, gy, gc,
                            gf, coef);
-    if (F.nnz)
-        hipLaunchKernelGGL(k_clip_grad, dim3(grid_for(F.nnz, 64)), dim3(64),
+    if (F.nnz) {
+        // most pairs by boundary decomposition; the pairs it declines through the clip pipeline
+        DevBuf<int32_t> list_b;
+        DevBuf<uint32_t> count_b;
+        int32_t *list = ws ? ws->list : nullptr;
+        uint32_t *count = ws ? ws->count : nullptr;
+        if (!ws) {
+            list_b.alloc_async(F.nnz, s);
+            count_b.alloc_async(1, s);
+            list = list_b.p;
+            count = count_b.p;
+        }
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, count);  // a kernel: no memset node in a graph
+        hipLaunchKernelGGL(k_clip_grad_edges, dim3(grid_for(F.nnz, 64)), dim3(64),
                            (size_t)kGradRows * dst->nv * 3 * sizeof(double), s, rowid, F.col.p, coef, F.nnz,
                            dst->cells.p, dst->nv, dst->vidx.p, src->cells.p, src->nv, src->vidx.p,
-                           R->stats.clip_with_dst, g_dst_verts, g_src_verts);
+                           R->stats.clip_with_dst, g_dst_verts, g_src_verts, list, count);
+        hipLaunchKernelGGL(k_clip_grad_list, dim3(kListBlocks), dim3(64),
+                           (size_t)kGradRows * dst->nv * 3 * sizeof(double), s, rowid, F.col.p, coef,
+                           dst->cells.p, dst->nv, dst->vidx.p, src->cells.p, src->nv, src->vidx.p,
+                           R->stats.clip_with_dst, g_dst_verts, g_src_verts, (const int32_t *)list,
+                           (const uint32_t *)count, F.nrows);
+        CR_HIP(hipGetLastError());
+    }
     double *g_to = transpose ? g_src_verts : g_dst_verts;
     const cr_grid *gto = transpose ? src : dst;
     if (normalize && g_to) {
-        if (gto->private_vertices && gto->nv <= kStagedNv)
+        const bool table = gto->private_vertices && gto->flipped == 0;  // slot -> vertex is the identity
+        if (table && gto->nv == 4)
+            hipLaunchKernelGGL(k_area_grad_table<4>, dim3(grid_for(n_to, 64)), dim3(64), 0, s, gto->cells.p, n_to,
+                               kappa, a_to, nfields, y, yc, yf, gy, gc, gf, g_to);
+        else if (table && gto->nv == 8)
+            hipLaunchKernelGGL(k_area_grad_table<8>, dim3(grid_for(n_to, 64)), dim3(64), 0, s, gto->cells.p, n_to,
+                               kappa, a_to, nfields, y, yc, yf, gy, gc, gf, g_to);
+        else if (table && gto->nv == 16)
+            hipLaunchKernelGGL(k_area_grad_table<16>, dim3(grid_for(n_to, 64)), dim3(64), 0, s, gto->cells.p,
+                               n_to, kappa, a_to, nfields, y, yc, yf, gy, gc, gf, g_to);
+        else if (gto->private_vertices && gto->nv <= kStagedNv)
             hipLaunchKernelGGL(k_area_grad_private, dim3(grid_for(n_to, 64)), dim3(64),
                                2 * 64 * 3 * gto->nv * sizeof(double), s, gto->cells.p, gto->nv,
                                gto->vidx.p, n_to, kappa, a_to, nfields, y, yc, yf, gy, gc, gf, g_to);
@@ -414,8 +658,10 @@ cr_status cr_regrid_backward_noalloc(const cr_regridder *R, const cr_grid *dst, 
         const bool verts = g_dst_verts || g_src_verts;
         const size_t cbytes = verts ? up((size_t)F.nnz * sizeof(double)) : 0;
         const size_t rbytes = verts ? up((size_t)F.nnz * sizeof(int32_t)) : 0;
-        CR_REQUIRE(wbytes + cbytes + rbytes <= kNoAllocScratchBytes, CR_INVALID_ARGUMENT,
-                   "cr_regrid_backward_noalloc: the call needs " + std::to_string(wbytes + cbytes + rbytes) +
+        const size_t lbytes = verts ? up((size_t)F.nnz * sizeof(int32_t)) + 256 : 0;  // list + count
+        const size_t need = wbytes + cbytes + rbytes + lbytes;
+        CR_REQUIRE(need <= kNoAllocScratchBytes, CR_INVALID_ARGUMENT,
+                   "cr_regrid_backward_noalloc: the call needs " + std::to_string(need) +
                        " bytes of scratch, over its maximum of " + std::to_string(kNoAllocScratchBytes) +
                        " (use cr_regrid_backward)");
         unsigned char *base = scratch_address();
@@ -423,6 +669,8 @@ cr_status cr_regrid_backward_noalloc(const cr_regridder *R, const cr_grid *dst, 
         ws.w = wbytes ? (double *)base : nullptr;
         ws.coef = cbytes ? (double *)(base + wbytes) : nullptr;
         ws.rowid = rbytes ? (int32_t *)(base + wbytes + cbytes) : nullptr;
+        ws.count = lbytes ? (uint32_t *)(base + wbytes + cbytes + rbytes) : nullptr;
+        ws.list = lbytes ? (int32_t *)(base + wbytes + cbytes + rbytes + 256) : nullptr;
         backward(R, dst, src, transpose, normalize, nfields, x, xc, xf, y, yc, yf, gy, gc, gf, gx, gxc, gxf,
                  g_dst_verts, g_src_verts, as_stream(stream), &ws);
     });

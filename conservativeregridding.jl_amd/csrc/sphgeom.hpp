@@ -455,6 +455,25 @@ CR_HD void clip_area_backward(const double *clip, int nc, const double *subj, in
     P.area_close();
 }
 
+// crossing_backward for a crossing that lies on the arc p1 -> q1, with the
+// gradient already made tangential at the crossing: the sign of y is that of
+// y.(p1 + q1).
+CR_HD bool crossing_backward_arc(const d3 &gt, const d3 &p1, const d3 &q1, const d3 &p2, const d3 &q2, d3 &gp1,
+                                 d3 &gq1, d3 &gp2, d3 &gq2) {
+    d3 n1 = cross(p1, q1), n2 = cross(p2, q2);
+    d3 y = cross(n1, n2);
+    double L = sqrt(dot(y, y));
+    if (!(L > 1e-300)) return false;
+    double sigma = dot(y, p1 + q1) >= 0.0 ? 1.0 : -1.0;
+    d3 yb = (sigma / L) * gt;
+    d3 n1b = cross(n2, yb), n2b = cross(yb, n1);
+    gp1 = cross(q1, n1b);
+    gq1 = cross(n1b, p1);
+    gp2 = cross(q2, n2b);
+    gq2 = cross(n2b, p2);
+    return true;
+}
+
 // Gradient w.r.t. the great-circle crossing x = y/|y| (y = +-(n1 x n2),
 // n1 = p1 x q1, n2 = p2 x q2) pushed back to p1, q1, p2, q2.
 CR_HD bool crossing_backward(const d3 &x, const d3 &g, const d3 &p1, const d3 &q1, const d3 &p2,
@@ -471,6 +490,393 @@ CR_HD bool crossing_backward(const d3 &x, const d3 &g, const d3 &p1, const d3 &q
     gp2 = cross(q2, n2b);
     gq2 = cross(n2b, p2);
     return true;
+}
+
+// ============================================================================
+// Backward by boundary decomposition (no clip pipeline): edges_plan, then
+// edges_emit.
+//
+// area(P), P = subject ∩ clip, is the sum over P's boundary arcs a -> b of the
+// fan triangle E(O, a, b) for any fixed point O (the area does not depend on
+// it; O is taken on P's boundary so that every triangle is as small as P).
+// The arcs are (i) the part of each subject edge inside the convex clip
+// polygon -- one interval, from the edge's side values against every clip
+// line: the latest entry and the earliest exit (Liang-Barsky on the chord
+// parameter; the crossings are the forward's crossing() of the edge's own end
+// points) -- and (ii) the parts of the clip edges inside the subject, which
+// begin where a subject edge leaves the clip polygon through that edge's line
+// or at a clip vertex inside the subject, and end where a subject edge enters
+// or at the next clip vertex.  A clip edge carries at most two such points
+// here.  Each arc's two end points take their derivatives: a subject vertex
+// or clip vertex directly, a crossing through crossing_backward (labels as in
+// clip_area_backward, g not yet tangential, coef applied).  Every loop runs
+// over the ring sizes: no clip stages, no recursion.
+//
+// edges_plan returns false (and edges_emit then emits nothing for the pair)
+// when the pair is near a configuration where the area is not differentiable
+// or where the decisions above could differ from the forward clip's by
+// rounding -- a subject vertex within ~1e-10 of a clip edge's length from
+// that edge (on its line between the lines through its corners), a crossing
+// within that distance of a clip corner or of another crossing on its clip
+// edge, an edge entering and leaving through neighbouring lines at one point
+// -- when a clip edge meets the subject's boundary more than twice, or when a
+// clip edge is degenerate.  The caller then uses clip_area_backward.
+// ============================================================================
+constexpr double kEdgeTol = 1e-10;
+
+// d E(O, a, b) / da and / db for the fan triangle O -> a -> b (O fixed).
+CR_HD void fan_grad(const d3 &O, const d3 &a, const d3 &b, double coef, d3 &ga, d3 &gb) {
+    const double T = dot(O, cross(a, b));
+    const double D = 1.0 + dot(O, a) + dot(a, b) + dot(b, O);
+    const double f = coef * 2.0 / (T * T + D * D);
+    ga = f * (D * cross(b, O) - T * (O + b));
+    gb = f * (D * cross(O, a) - T * (O + a));
+}
+
+// The crossings of one subject edge with the clip lines: the latest entry
+// (k_in, t_in) and the earliest exit (k_out, t_out); empty when an end point
+// pair lies outside one line.
+// The winners' side values (and, with corners, their clip edge's end points)
+// are carried out of the unrolled loop, so no array is indexed by a run-time
+// line number (which would put it in scratch memory).
+template <int NC>
+struct EdgeCut {
+    double t_in, t_out, ap_in, aq_in, ap_out, aq_out;
+    // POS: the end points' side values against the lines before and after
+    // the winning line (through its clip edge's first and last corner), and
+    // their tolerances: at the crossing they place it along the clip edge
+    double bp_in, bq_in, fp_in, fq_in, btol_in, ftol_in;
+    double bp_out, bq_out, fp_out, fq_out, btol_out, ftol_out;
+    int k_in, k_out;
+    bool empty;
+    template <bool POS>
+    CR_HD void make(const double (&ap)[NC], const double (&aq)[NC], const double (&tol)[NC]) {
+        t_in = 0.0;
+        t_out = 1.0;
+        ap_in = aq_in = ap_out = aq_out = 0.0;
+        k_in = -1;
+        k_out = -1;
+        empty = false;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            const bool pin = ap[k] >= 0.0, qin = aq[k] >= 0.0;
+            if (pin && qin) continue;
+            if (!pin && !qin) {
+                empty = true;
+                continue;
+            }
+            const double t = fabs(ap[k]) / (fabs(ap[k]) + fabs(aq[k]));
+            const int b = (k + NC - 1) % NC, f = (k + 1) % NC;
+            if (!pin && t > t_in) {
+                t_in = t;
+                k_in = k;
+                ap_in = ap[k];
+                aq_in = aq[k];
+                if (POS) {
+                    bp_in = ap[b];
+                    bq_in = aq[b];
+                    fp_in = ap[f];
+                    fq_in = aq[f];
+                    btol_in = tol[b];
+                    ftol_in = tol[f];
+                }
+            }
+            if (pin && t < t_out) {
+                t_out = t;
+                k_out = k;
+                ap_out = ap[k];
+                aq_out = aq[k];
+                if (POS) {
+                    bp_out = ap[b];
+                    bq_out = aq[b];
+                    fp_out = ap[f];
+                    fq_out = aq[f];
+                    btol_out = tol[b];
+                    ftol_out = tol[f];
+                }
+            }
+        }
+    }
+    CR_HD bool inside() const { return !empty && t_in < t_out; }
+};
+
+// The crossing of subject edge j with the clip line through ck with normal
+// nk: the forward's crossing() of the edge's own end points.
+CR_HD d3 edge_cut_point(const double *subj, int ns, int j, const d3 &ck, const d3 &nk) {
+    const d3 p = load3(subj + 3 * j), q = load3(subj + 3 * (j + 1 == ns ? 0 : j + 1));
+    return crossing(p, dot(nk, p - ck), q, dot(nk, q - ck));
+}
+
+// What pass 1 leaves for pass 2: the clip ring and its lines, the crossings
+// on every clip edge (at most two, in order along the edge; kind 1 = a
+// clip-edge arc begins there), and O.
+template <int NC>
+struct EdgesPlan {
+    d3 c[NC], n[NC], O;
+    int32_t lab_lo[NC], lab_hi[NC];
+    int kind_lo[NC], kind_hi[NC], cnt[NC];
+    bool any_out;
+};
+
+// Pass 1: the checks (false: use clip_area_backward) and the plan.
+template <int NC>
+CR_HD bool edges_plan_(const double *clip, int nc, const double *subj, int ns, EdgesPlan<NC> &P) {
+    double tol[NC];
+    // a declined plan still drives edges_emit in step with the other lanes
+    // (emitting nothing): it reads no crossing labels (no counts) and finite lines
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        P.c[k] = load3(clip + 3 * (k < nc ? k : nc - 1));
+        P.n[k] = mk3(0.0, 0.0, 0.0);
+        P.lab_lo[k] = P.lab_hi[k] = -1;
+        P.kind_lo[k] = P.kind_hi[k] = 0;
+        P.cnt[k] = 0;
+    }
+    P.any_out = false;
+    P.O = load3(subj);
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        const d3 e = P.c[(k + 1) % NC] - P.c[k];
+        const d3 nk = cross(P.c[k], e);
+        const double nn = dot(nk, nk);
+        // a degenerate clip edge (a polar cell's coincident corners, a padded
+        // triangle) is no constraint in the forward clip, whose corner there is
+        // the crossing of the two neighbouring lines: clip_area_backward
+        // differentiates that corner; this decomposition would not
+        if (nn < kDegenerateEdge2) return false;
+        P.n[k] = nk;
+        tol[k] = kEdgeTol * nn;  // |side| below this: within ~1e-10 edge lengths of the line
+    }
+    double u_lo[NC], u_hi[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) u_lo[k] = u_hi[k] = -1.0;
+    bool have_O = false, overflow = false;
+    // a subject vertex on a clip line is a degenerate point only on the
+    // clip edge's segment (inside the lines through its two corners); off it,
+    // both clips take the line's side test the same way
+    auto on_edge = [&](const double (&a)[NC]) {
+        bool hit = false;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            const int b = (k + NC - 1) % NC, f = (k + 1) % NC;
+            hit = hit || (fabs(a[k]) <= tol[k] && a[b] >= -tol[b] && a[f] >= -tol[f]);
+        }
+        return hit;
+    };
+    d3 p = load3(subj);
+    double ap[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) ap[k] = dot(P.n[k], p - P.c[k]);
+    if (on_edge(ap)) return false;
+    for (int j = 0; j < ns; ++j) {
+        const d3 q = load3(subj + 3 * (j + 1 == ns ? 0 : j + 1));
+        double aq[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            aq[k] = dot(P.n[k], q - P.c[k]);
+            P.any_out = P.any_out || aq[k] < 0.0;
+        }
+        if (on_edge(aq)) return false;
+        EdgeCut<NC> cut;
+        cut.template make<true>(ap, aq, tol);
+        if (cut.k_in >= 0 && cut.k_out >= 0 && fabs(cut.t_in - cut.t_out) <= kEdgeTol) return false;
+        if (cut.inside()) {
+            if (!have_O) {
+                P.O = cut.k_in >= 0 ? crossing(p, cut.ap_in, q, cut.aq_in) : p;
+                have_O = true;
+            }
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const int k = side == 0 ? cut.k_in : cut.k_out;
+                if (k < 0) continue;
+                // the crossing's place along clip edge k: its side values (linear
+                // along the chord) against the line through the edge's first
+                // corner (0 there, growing along the edge) and through its last
+                const double t = side == 0 ? cut.t_in : cut.t_out;
+                const double u = side == 0 ? (1.0 - t) * cut.bp_in + t * cut.bq_in
+                                           : (1.0 - t) * cut.bp_out + t * cut.bq_out;
+                const double v = side == 0 ? (1.0 - t) * cut.fp_in + t * cut.fq_in
+                                           : (1.0 - t) * cut.fp_out + t * cut.fq_out;
+                if (u <= (side == 0 ? cut.btol_in : cut.btol_out) || v <= (side == 0 ? cut.ftol_in : cut.ftol_out))
+                    return false;  // within ~1e-10 of a clip corner
+                // entering the clip polygon through line k ends a clip-edge arc there (kind 0),
+                // leaving through it begins one (kind 1)
+                const int32_t lab = lab_make(kLabXF, j, k);
+#pragma unroll
+                for (int m = 0; m < NC; ++m) {
+                    if (m != k) continue;
+                    if (P.cnt[m] == 0) {
+                        u_lo[m] = u;
+                        P.lab_lo[m] = lab;
+                        P.kind_lo[m] = side;
+                    } else if (P.cnt[m] == 1) {
+                        if (fabs(u - u_lo[m]) <= tol[(m + NC - 1) % NC]) overflow = true;
+                        if (u > u_lo[m]) {
+                            u_hi[m] = u;
+                            P.lab_hi[m] = lab;
+                            P.kind_hi[m] = side;
+                        } else {
+                            u_hi[m] = u_lo[m];
+                            P.lab_hi[m] = P.lab_lo[m];
+                            P.kind_hi[m] = P.kind_lo[m];
+                            u_lo[m] = u;
+                            P.lab_lo[m] = lab;
+                            P.kind_lo[m] = side;
+                        }
+                    } else {
+                        overflow = true;
+                    }
+                    P.cnt[m] += 1;
+                }
+            }
+        }
+        p = q;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) ap[k] = aq[k];
+    }
+    if (overflow) return false;
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+        if (P.cnt[k] == 2 && P.kind_lo[k] == P.kind_hi[k]) return false;  // not an arc's two ends
+    if (!have_O) P.O = P.any_out ? P.c[0] : load3(subj);  // P is the clip polygon, or the subject
+    return true;
+}
+
+template <int NC>
+CR_HD bool edges_plan(const double *clip, int nc, const double *subj, int ns, EdgesPlan<NC> &P) {
+    if (edges_plan_<NC>(clip, nc, subj, ns, P)) return true;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) P.cnt[k] = 0;
+    return false;
+}
+
+// The arcs of clip edge K: the state at c_K (inside the subject or not) is
+// carried from the last crossing before it along the clip boundary; with no
+// crossing anywhere P is the clip polygon (some subject vertex outside it) or
+// the subject.  Clip-vertex parts go to gc, crossings to emit.
+template <int K, int NC, class Emit>
+CR_HD void clip_arcs(const EdgesPlan<NC> &P, const double *subj, int ns, double coef, bool any_ev,
+                     d3 (&gc)[NC], Emit &emit) {
+    constexpr int K1 = (K + 1) % NC;
+    const d3 &O = P.O;
+    int in_start = P.any_out ? 1 : 0;
+    if (any_ev) {
+        bool found = false;
+#pragma unroll
+        for (int s = 1; s <= NC; ++s) {
+            const int m = (K - s + NC) % NC;
+            if (!found && P.cnt[m] > 0) {
+                in_start = P.cnt[m] == 2 ? P.kind_hi[m] : P.kind_lo[m];
+                found = true;
+            }
+        }
+    }
+    if (P.cnt[K] == 0) {
+        if (in_start) {
+            d3 ga, gb;
+            fan_grad(O, P.c[K], P.c[K1], coef, ga, gb);
+            gc[K] = gc[K] + ga;
+            gc[K1] = gc[K1] + gb;
+        }
+        return;
+    }
+    const int32_t l1 = P.lab_lo[K];
+    const d3 x1 = edge_cut_point(subj, ns, lab_hi(l1), P.c[K], P.n[K]);
+    if (P.kind_lo[K] == 0) {  // an arc from c_K ends at the first crossing
+        d3 ga, gb;
+        fan_grad(O, P.c[K], x1, coef, ga, gb);
+        gc[K] = gc[K] + ga;
+        emit(l1, x1, gb);
+    }
+    if (P.cnt[K] == 1) {
+        if (P.kind_lo[K] == 1) {  // an arc begins there and runs to c_K+1
+            d3 ga, gb;
+            fan_grad(O, x1, P.c[K1], coef, ga, gb);
+            emit(l1, x1, ga);
+            gc[K1] = gc[K1] + gb;
+        }
+        return;
+    }
+    const int32_t l2 = P.lab_hi[K];
+    const d3 x2 = edge_cut_point(subj, ns, lab_hi(l2), P.c[K], P.n[K]);
+    d3 ga, gb;
+    if (P.kind_lo[K] == 1) {  // one arc between the two crossings
+        fan_grad(O, x1, x2, coef, ga, gb);
+        emit(l1, x1, ga);
+        emit(l2, x2, gb);
+    } else {  // the second crossing begins an arc to c_K+1
+        fan_grad(O, x2, P.c[K1], coef, ga, gb);
+        emit(l2, x2, ga);
+        gc[K1] = gc[K1] + gb;
+    }
+}
+
+template <int K, int NC, class Emit, class Sync>
+CR_HD void clip_arcs_from(const EdgesPlan<NC> &P, const double *subj, int ns, double coef, bool any_ev,
+                          d3 (&gc)[NC], Emit &emit, Sync &sync) {
+    clip_arcs<K, NC>(P, subj, ns, coef, any_ev, gc, emit);
+    sync();
+    if constexpr (K + 1 < NC) clip_arcs_from<K + 1, NC>(P, subj, ns, coef, any_ev, gc, emit, sync);
+}
+
+// Pass 2: every arc's end-point derivatives through emit.  sync() is called
+// after every subject edge and every clip edge, the same number of times
+// whatever the pair (a caller running lanes in step may act there).
+template <int NC, class Emit, class Sync>
+CR_HD void edges_emit(const EdgesPlan<NC> &P, const double *subj, int ns, double coef, Emit &emit, Sync &&sync) {
+    const d3 O = P.O;
+    // the subject arcs
+    const int32_t labS = lab_make(kLabS, 0, 0);
+    d3 p = load3(subj), g_first = mk3(0.0, 0.0, 0.0), g_pend = mk3(0.0, 0.0, 0.0);
+    double ap[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) ap[k] = dot(P.n[k], p - P.c[k]);
+    for (int j = 0; j < ns; ++j) {
+        const d3 q = load3(subj + 3 * (j + 1 == ns ? 0 : j + 1));
+        double aq[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) aq[k] = dot(P.n[k], q - P.c[k]);
+        EdgeCut<NC> cut;
+        cut.template make<false>(ap, aq, ap);  // no positions: the third argument is not read
+        d3 g_p = mk3(0.0, 0.0, 0.0), g_q = mk3(0.0, 0.0, 0.0);  // this edge's parts of s_j and s_j+1
+        if (cut.inside()) {
+            const d3 xa = cut.k_in >= 0 ? crossing(p, cut.ap_in, q, cut.aq_in) : p;
+            const d3 xb = cut.k_out >= 0 ? crossing(p, cut.ap_out, q, cut.aq_out) : q;
+            d3 ga, gb;
+            fan_grad(O, xa, xb, coef, ga, gb);
+            if (cut.k_in >= 0) emit(lab_make(kLabXF, j, cut.k_in), xa, ga);
+            else g_p = ga;
+            if (cut.k_out >= 0) emit(lab_make(kLabXF, j, cut.k_out), xb, gb);
+            else g_q = gb;
+        }
+        if (j == 0) {
+            g_first = g_p;
+        } else {
+            const d3 g = g_pend + g_p;  // subject vertex j: from edges j - 1 and j
+            if (g.x != 0.0 || g.y != 0.0 || g.z != 0.0) emit(labS | j, p, g);
+        }
+        g_pend = g_q;
+        p = q;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) ap[k] = aq[k];
+        sync();
+    }
+    {
+        const d3 g = g_first + g_pend;  // vertex 0: from the first and the closing edge
+        if (g.x != 0.0 || g.y != 0.0 || g.z != 0.0) emit(labS, p, g);
+    }
+    // the clip arcs, one clip edge at a time (K a template argument: every
+    // index stays compile-time whatever the compiler unrolls)
+    bool any_ev = false;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) any_ev = any_ev || P.cnt[k] > 0;
+    d3 gc[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) gc[k] = mk3(0.0, 0.0, 0.0);
+    clip_arcs_from<0, NC>(P, subj, ns, coef, any_ev, gc, emit, sync);
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+        if (gc[k].x != 0.0 || gc[k].y != 0.0 || gc[k].z != 0.0) emit(lab_make(kLabC, 0, k), P.c[k], gc[k]);
 }
 
 // coef * d(signed ring area)/d(vertices) through emit(slot, grad): one call per
@@ -494,6 +900,28 @@ CR_HD void ring_area_backward(const double *xyz, int n, double coef, Emit &&emit
     }
     emit(n - 1, tangential(pend, b));
     emit(0, tangential(ga, a));
+}
+
+// ring_area_backward for rings of NV vertices, the slot gradients added to
+// g[NV] (tangential; unrolled, so the slots are compile-time indices).
+template <int NV>
+CR_HD void ring_area_backward_n(const double *xyz, double coef, d3 (&g)[NV]) {
+    const d3 a = load3(xyz);
+    d3 ga = mk3(0, 0, 0), pend = mk3(0, 0, 0);
+    d3 b = load3(xyz + 3);
+#pragma unroll
+    for (int i = 1; i + 1 < NV; ++i) {
+        const d3 c = load3(xyz + 3 * (i + 1));
+        const double T = dot(a, cross(b, c));
+        const double D = 1.0 + dot(a, b) + dot(b, c) + dot(c, a);
+        const double f = coef * 2.0 / (T * T + D * D);
+        ga = ga + f * (D * cross(b, c) - T * (b + c));
+        g[i] = g[i] + tangential(pend + f * (D * cross(c, a) - T * (a + c)), b);
+        pend = f * (D * cross(a, b) - T * (a + b));
+        b = c;
+    }
+    g[NV - 1] = g[NV - 1] + tangential(pend, b);
+    g[0] = g[0] + tangential(ga, a);
 }
 
 }  // namespace cr
