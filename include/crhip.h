@@ -314,9 +314,11 @@ cr_status cr_regrid_backward(const cr_regridder *R, const cr_grid *dst, const cr
  * cr_regrid_backward without allocating (no reference counterpart: for callers
  * that cannot allocate device memory -- code under HIP graph capture, tight
  * loops, harnesses that forbid hipMalloc).  Same arguments and results; its
- * scratch (n_to * nfields doubles for gx, 12 bytes per non-zero for vertex
- * gradients, each rounded up to 256 bytes; n_to = the output cell count of the
- * direction) lives in one static device array of 64 MiB per GPU, kept until
+ * scratch (n_to * nfields doubles for gx; for vertex gradients 16 bytes per
+ * non-zero and 256 more (a coefficient, a row and a list slot per non-zero,
+ * and the list's count); each part rounded up to 256 bytes; n_to = the output
+ * cell count of the direction) lives in one static device array of 64 MiB per
+ * GPU, kept until
  * the process ends.  Larger requests fail with CR_INVALID_ARGUMENT.  When gx is
  * wanted, the field adjoint's plan must exist: call
  * cr_regrid_prepare(R, !transpose, CR_F64, nfields, n_to, gx's field stride)
