@@ -164,7 +164,10 @@ cr_status cr_validate_points(const double *pts, int64_t nblocks, int64_t ni, int
 cr_status cr_grid_from_cells(cr_grid **grid, const double *cells, int64_t ncells, int32_t nv,
                              int32_t convex, void *stream);
 /* Upload point blocks (a lon-lat Matrix{UnitSphericalPoint} is one block, a
- * cubed sphere six) and expand them to cells on the device. */
+ * cubed sphere six) and expand them to cells on the device.  The points are
+ * checked on the host before anything reaches the device, the cells' geometry
+ * on the device after the upload; the errors of cr_validate_points, naming the
+ * same lowest cell. */
 cr_status cr_grid_from_points(cr_grid **grid, const double *pts, int64_t nblocks, int64_t ni,
                               int64_t nj, void *stream);
 /* Upload a ragged polygon list (layout of cr_polygons_to_cells) with shared

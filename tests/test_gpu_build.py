@@ -182,9 +182,10 @@ def test_zero_area_cells_are_counted(cr, orc):
 
 
 def test_point_grid_cells_are_checked_on_the_device(cr):
-    """Grid.from_points checks the points on the host and the cells' geometry
-    on the device: the same errors, naming the same lowest cell, as the host
-    check cr_validate_points."""
+    """Grid.from_points checks the points on the host (a point that is not a
+    finite unit vector fails before anything reaches the device) and the
+    cells' geometry on the device: the same errors, naming the same lowest
+    cell, as the host check cr_validate_points."""
     import ctypes
     import re
     _print_case("device_point_checks")
@@ -201,13 +202,25 @@ def test_point_grid_cells_are_checked_on_the_device(cr):
     concave = base.copy()
     v = concave[7, 5] + 0.8 * (concave[8, 6] - concave[7, 5])  # a corner pulled across its cell
     concave[7, 5] = v / np.linalg.norm(v)
+    cell = lambda m: re.search(r"cell (\d+)", m).group(1)  # noqa: E731
     for P in (anti, concave):
         st, msg = host(P)
         assert st == L.CR_DEGENERATE_GEOMETRY
         with pytest.raises(cr.CRError) as e:
             cr.Grid.from_points(P)
         assert e.value.status == L.CR_DEGENERATE_GEOMETRY
-        cell = lambda m: re.search(r"cell (\d+)", m).group(1)  # noqa: E731
         assert cell(msg) == cell(str(e.value))
+    for where, how in (((0, 0), "scaled"), ((20, 9), "nan"), ((36, 18), "inf"), ((5, 0), "scaled")):
+        P = base.copy()
+        if how == "scaled":
+            P[where] *= 1.001
+        else:
+            P[where][1] = np.nan if how == "nan" else np.inf
+        P[30, 10] = -P[31, 10]  # a later geometry error: the vertex error is named first
+        st, msg = host(P)
+        assert st == L.CR_INVALID_ARGUMENT
+        with pytest.raises(cr.ArgumentError) as e:
+            cr.Grid.from_points(P)
+        assert "unit vector" in str(e.value) and cell(msg) == cell(str(e.value)), (where, how)
     assert host(base)[0] == 0
     cr.Grid.from_points(base)

@@ -4,7 +4,8 @@
   rejected (CR_INVALID_ARGUMENT), antipodal edges and non-convex cells of a
   convex grid are degenerate (CR_DEGENERATE_GEOMETRY), cells collapsed to one
   point are legal and counted; every grid the configurations use passes.
-* The grid constructors run the same checks before touching the device.
+* The grid constructors run the same checks before touching the device
+  (point matrices: their points; their cells on the device after the upload).
 * Regridder(dst, src) manifold resolution of src/regridder/regridder.jl:105-123.
 """
 import ctypes
@@ -78,7 +79,9 @@ def test_non_unit_or_nonfinite_vertex_is_rejected(cr, bad):
 
 
 def test_grid_constructors_check_before_the_device(cr):
-    """The checks run on the host first: the error is the geometry's, not CR_NO_DEVICE."""
+    """The checks run on the host first: the error is the geometry's, not
+    CR_NO_DEVICE (point matrices: their points; their cells' geometry is
+    checked on the device after the upload, tests/test_gpu_build.py)."""
     cells = np.stack([_lonlat_cell(cr, 0.0, 0.0, 4.0, 4.0)] * 2)
     cells[1, 2] = -cells[1, 1]
     with pytest.raises(cr.CRError) as e:
