@@ -47,7 +47,9 @@ __global__ void k_pair_coef(const int32_t *__restrict__ rowid, const int32_t *__
 
 // Scatter target of the clip backward: caller-vertex gradients of both grids.
 // The destination cell's slots may go to a block-local accumulator instead
-// (lacc_subj or lacc_clip, 3 doubles per slot), flushed once per block.
+// (lacc_subj or lacc_clip, 3 doubles per slot), flushed once per block.  NC:
+// the clip slots of the kernel (clip index NC wraps to 0).
+template <int NC>
 struct VertexSink {
     const double *subj, *clip;
     int ns, nc;
@@ -62,7 +64,7 @@ struct VertexSink {
     __device__ int wrap(int e) const { return e >= ns ? e - ns : e; }
     __device__ d3 sv(int e) const { return load3(subj + 3 * wrap(e)); }
     __device__ d3 cv(int k) const {
-        int kk = k % 4;
+        const int kk = k >= NC ? k - NC : k;
         return load3(clip + 3 * (kk < nc ? kk : nc - 1));
     }
     __device__ static void add(double *g, int32_t v, const d3 &d) {
@@ -94,7 +96,7 @@ struct VertexSink {
         }
     }
     __device__ void add_clip(int k, const d3 &g) const {
-        int kk = k % 4;
+        int kk = k >= NC ? k - NC : k;
         kk = kk < nc ? kk : nc - 1;
         const d3 t = tangential(g, cv(kk));
         if (lacc_clip) {
@@ -141,8 +143,10 @@ constexpr int kGradRows = 16;
 
 // The sink of one pair: its two rings, its slots' accumulators (the wave's
 // destination rows in LDS when `lacc` is given and the row is one of them,
-// the pair's own source-cell sums in `own`) and the caller-vertex gradients.
-__device__ __forceinline__ VertexSink pair_sink(int64_t d, int64_t s, int64_t r0, double *lacc, double *own,
+// the pair's own source-cell sums in `own`) and the caller-vertex gradients;
+// NC: the clip slots of the kernel.
+template <int NC>
+__device__ __forceinline__ VertexSink<NC> pair_sink(int64_t d, int64_t s, int64_t r0, double *lacc, double *own,
                                                 const double *__restrict__ dst_cells, int dst_nv,
                                                 const int32_t *__restrict__ dst_vi,
                                                 const double *__restrict__ src_cells, int src_nv,
@@ -151,20 +155,23 @@ __device__ __forceinline__ VertexSink pair_sink(int64_t d, int64_t s, int64_t r0
     const double *dp = dst_cells + 3 * (int64_t)dst_nv * d;
     const double *sp = src_cells + 3 * (int64_t)src_nv * s;
     double *mine = (lacc && g_dst && d - r0 < kGradRows) ? lacc + (d - r0) * dst_nv * 3 : nullptr;
-    if (clip_is_dst)  // a source subject of at most 4 vertices sums its slots per pair
-        return VertexSink{sp, dp, src_nv, dst_nv, src_vi + (int64_t)src_nv * s, dst_vi + (int64_t)dst_nv * d,
-                          g_src, g_dst, nullptr, mine, nullptr, (g_src && src_nv <= 4) ? own : nullptr};
-    return VertexSink{dp, sp, dst_nv, src_nv, dst_vi + (int64_t)dst_nv * d, src_vi + (int64_t)src_nv * s,
-                      g_dst, g_src, mine, nullptr, g_src ? own : nullptr, nullptr};
+    // a source cell of at most 4 vertices sums its slots per pair
+    double *own4 = (g_src && src_nv <= 4) ? own : nullptr;
+    return clip_is_dst
+                       ? VertexSink<NC>{sp, dp, src_nv, dst_nv, src_vi + (int64_t)src_nv * s, dst_vi + (int64_t)dst_nv * d,
+                                    g_src, g_dst, nullptr, mine, nullptr, own4}
+                       : VertexSink<NC>{dp, sp, dst_nv, src_nv, dst_vi + (int64_t)dst_nv * d, src_vi + (int64_t)src_nv * s,
+                                    g_dst, g_src, mine, nullptr, own4, nullptr};
 }
 
 // The per-pair sums of the source cell's 4 slots, one atomic per component.
-__device__ __forceinline__ void flush_own(const VertexSink &sink, int src_nv, double *own, double *g_src) {
+template <int NC>
+__device__ __forceinline__ void flush_own(const VertexSink<NC> &sink, int src_nv, double *own, double *g_src) {
     if (!sink.own_clip && !sink.own_subj) return;
     const int32_t *vi = sink.own_clip ? sink.clip_vi : sink.subj_vi;
     for (int kk = 0; kk < src_nv && kk < 4; ++kk) {
         const d3 v = mk3(own[3 * kk], own[3 * kk + 1], own[3 * kk + 2]);
-        if (v.x != 0.0 || v.y != 0.0 || v.z != 0.0) VertexSink::add(g_src, vi[kk], v);
+        if (v.x != 0.0 || v.y != 0.0 || v.z != 0.0) VertexSink<NC>::add(g_src, vi[kk], v);
     }
 }
 
@@ -193,8 +200,9 @@ struct XQueue {
     int n;
 };
 
+template <int NC>
 struct QueueEmit {
-    VertexSink *sink;
+    VertexSink<NC> *sink;
     XQueue *q;
     bool live;
     // the decomposition emits subject vertices, clip vertices and
@@ -225,6 +233,7 @@ struct QueueEmit {
 // pair the decomposition declines (near a degenerate configuration, or a
 // degenerate clip edge) is appended to `list` (wave by wave, in order) for
 // k_clip_grad_list.
+template <int NC>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_clip_grad_edges(
     const int32_t *__restrict__ rowid, const int32_t *__restrict__ col, const double *__restrict__ coef,
     int64_t nnz, const double *__restrict__ dst_cells, int dst_nv, const int32_t *__restrict__ dst_vi,
@@ -250,10 +259,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
     if (lane == 0) q.n = 0;
     __syncthreads();
     const double c = real ? coef[kk] : 0.0;
-    VertexSink sink = pair_sink(d, s, r0, lacc, my_own, dst_cells, dst_nv, dst_vi, src_cells, src_nv, src_vi,
+    VertexSink<NC> sink = pair_sink<NC>(d, s, r0, lacc, my_own, dst_cells, dst_nv, dst_vi, src_cells, src_nv, src_vi,
                                 clip_is_dst, g_dst, g_src);
-    EdgesPlan<4> P;
-    const bool planned = edges_plan<4>(sink.clip, sink.nc, sink.subj, sink.ns, P);
+    EdgesPlan<NC> P;
+    const bool planned = edges_plan<NC>(sink.clip, sink.nc, sink.subj, sink.ns, P);
     const bool live = c != 0.0 && planned;
     // declined pairs to the list: one atomic per wave, lanes in order
     {
@@ -270,7 +279,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
         const int L = q.lane[e];
         const int32_t l = q.lab[e];
         const d3 gt = mk3(q.g[3 * e], q.g[3 * e + 1], q.g[3 * e + 2]);
-        VertexSink sk = pair_sink(q.pd[L], q.ps[L], r0, lacc, own + 12 * L, dst_cells, dst_nv, dst_vi, src_cells,
+        VertexSink<NC> sk = pair_sink<NC>(q.pd[L], q.ps[L], r0, lacc, own + 12 * L, dst_cells, dst_nv, dst_vi, src_cells,
                                   src_nv, src_vi, clip_is_dst, g_dst, g_src);
         sk.own_atomic = true;
         const int a = lab_hi(l), b = lab_lo(l);
@@ -294,8 +303,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
         if (lane == 0) q.n = drain ? 0 : n;
         __syncthreads();
     };
-    QueueEmit em{&sink, &q, live};
-    edges_emit<4>(P, sink.subj, sink.ns, c, em, [&] { flush(false); });
+    QueueEmit<NC> em{&sink, &q, live};
+    edges_emit<NC>(P, sink.subj, sink.ns, c, em, [&] { flush(false); });
     flush(true);
     if (live) flush_own(sink, src_nv, my_own, g_src);
     __syncthreads();
@@ -317,6 +326,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
 // whole vertex pass on C3: 8.4 ms with the normals in registers, 7.1 ms as
 // here, 9.1 ms with none spilled), and so did one wave per SIMD with no
 // spills (8.3 ms against 5.4).
+template <int NC>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_clip_grad_list(
     const int32_t *__restrict__ rowid, const int32_t *__restrict__ col, const double *__restrict__ coef,
     const double *__restrict__ dst_cells, int dst_nv, const int32_t *__restrict__ dst_vi,
@@ -325,7 +335,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
     const uint32_t *__restrict__ count, int64_t n_rows) {
     extern __shared__ double lacc[];  // kGradRows * dst_nv * 3
     __shared__ double own[64 * 12];
-    __shared__ d3 nrm_lds[64 * 4];  // each thread's 4 clip-edge normals
+    __shared__ d3 nrm_lds[64 * NC];  // each thread's NC clip-edge normals
     double *my_own = own + 12 * threadIdx.x;
     const int64_t total = *count;
     const int nacc = kGradRows * dst_nv * 3;
@@ -341,10 +351,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
             const double c = coef[k];
             const int64_t d = rowid[k];
             if (c != 0.0) {
-                VertexSink sink = pair_sink(d, col[k], r0, d >= r0 ? lacc : nullptr, my_own, dst_cells, dst_nv,
+                VertexSink<NC> sink = pair_sink<NC>(d, col[k], r0, d >= r0 ? lacc : nullptr, my_own, dst_cells, dst_nv,
                                             dst_vi, src_cells, src_nv, src_vi, clip_is_dst, g_dst, g_src);
-                clip_area_backward<4, VertexSink>(sink.clip, sink.nc, sink.subj, sink.ns, c, sink,
-                                                  nrm_lds + 4 * threadIdx.x);
+                clip_area_backward<NC, VertexSink<NC>>(sink.clip, sink.nc, sink.subj, sink.ns, c, sink,
+                                                   nrm_lds + NC * threadIdx.x);
                 flush_own(sink, src_nv, my_own, g_src);
             }
         }
@@ -547,7 +557,7 @@ static void backward(const cr_regridder *R, const cr_grid *dst, const cr_grid *s
     CR_REQUIRE(!dst->planar && !src->planar, CR_UNSUPPORTED,
                "vertex gradients are of spherical intersection areas; these grids are planar");
     const int clip_nv = R->stats.clip_with_dst ? dst->nv : src->nv;
-    CR_REQUIRE(clip_nv <= 4, CR_UNSUPPORTED, "vertex gradients need clip cells with at most 4 vertices");
+    CR_REQUIRE(clip_nv <= 8, CR_UNSUPPORTED, "vertex gradients need clip cells with at most 8 vertices");
     if (normalize) CR_REQUIRE(y, CR_INVALID_ARGUMENT, "cr_regrid_backward: the forward output y is needed");
     const double kappa = R->area_scale;
     DevBuf<double> coef_b;
@@ -578,15 +588,24 @@ static void backward(const cr_regridder *R, const cr_grid *dst, const cr_grid *s
             count = count_b.p;
         }
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, count);  // a kernel: no memset node in a graph
-        hipLaunchKernelGGL(k_clip_grad_edges, dim3(grid_for(F.nnz, 64)), dim3(64),
-                           (size_t)kGradRows * dst->nv * 3 * sizeof(double), s, rowid, F.col.p, coef, F.nnz,
-                           dst->cells.p, dst->nv, dst->vidx.p, src->cells.p, src->nv, src->vidx.p,
-                           R->stats.clip_with_dst, g_dst_verts, g_src_verts, list, count);
-        hipLaunchKernelGGL(k_clip_grad_list, dim3(kListBlocks), dim3(64),
-                           (size_t)kGradRows * dst->nv * 3 * sizeof(double), s, rowid, F.col.p, coef,
-                           dst->cells.p, dst->nv, dst->vidx.p, src->cells.p, src->nv, src->vidx.p,
-                           R->stats.clip_with_dst, g_dst_verts, g_src_verts, (const int32_t *)list,
-                           (const uint32_t *)count, F.nrows);
+        const size_t lds = (size_t)kGradRows * dst->nv * 3 * sizeof(double);
+        if (clip_nv <= 4) {
+            hipLaunchKernelGGL(k_clip_grad_edges<4>, dim3(grid_for(F.nnz, 64)), dim3(64), lds, s, rowid, F.col.p,
+                               coef, F.nnz, dst->cells.p, dst->nv, dst->vidx.p, src->cells.p, src->nv, src->vidx.p,
+                               R->stats.clip_with_dst, g_dst_verts, g_src_verts, list, count);
+            hipLaunchKernelGGL(k_clip_grad_list<4>, dim3(kListBlocks), dim3(64), lds, s, rowid, F.col.p, coef,
+                               dst->cells.p, dst->nv, dst->vidx.p, src->cells.p, src->nv, src->vidx.p,
+                               R->stats.clip_with_dst, g_dst_verts, g_src_verts, (const int32_t *)list,
+                               (const uint32_t *)count, F.nrows);
+        } else {
+            hipLaunchKernelGGL(k_clip_grad_edges<8>, dim3(grid_for(F.nnz, 64)), dim3(64), lds, s, rowid, F.col.p,
+                               coef, F.nnz, dst->cells.p, dst->nv, dst->vidx.p, src->cells.p, src->nv, src->vidx.p,
+                               R->stats.clip_with_dst, g_dst_verts, g_src_verts, list, count);
+            hipLaunchKernelGGL(k_clip_grad_list<8>, dim3(kListBlocks), dim3(64), lds, s, rowid, F.col.p, coef,
+                               dst->cells.p, dst->nv, dst->vidx.p, src->cells.p, src->nv, src->vidx.p,
+                               R->stats.clip_with_dst, g_dst_verts, g_src_verts, (const int32_t *)list,
+                               (const uint32_t *)count, F.nrows);
+        }
         CR_HIP(hipGetLastError());
     }
     double *g_to = transpose ? g_src_verts : g_dst_verts;

@@ -305,7 +305,9 @@ cr_status cr_regrid_prepare(const cr_regridder *R, int32_t transpose, int32_t dt
  *   cr_grid_vertex_count; ACCUMULATED with FP64 atomics, may be NULL) = dL/dV
  *   through every intersection area and the destination cell areas.
  * dst and src must be the grids R was built from (vertex gradients are not
- * available for cr_regridder_from_csr regridders).  y is needed when normalize=1.
+ * available for cr_regridder_from_csr regridders, for planar grids, or when
+ * the clipping side's cells have more than 8 vertices: CR_UNSUPPORTED).  y is
+ * needed when normalize=1.
  */
 cr_status cr_regrid_backward(const cr_regridder *R, const cr_grid *dst, const cr_grid *src,
                              int32_t transpose, int32_t normalize, int64_t nfields, const double *x,

@@ -137,3 +137,46 @@ def test_ragged_vertex_gradient(cr, orc):
             scale = max(abs(fd), np.abs(g_src[p]).max(), 1e-3)
             worst = max(worst, abs(g_src[p, k] - fd) / scale)
     assert worst < 1e-5, worst
+
+
+def test_hexagon_clip_side_gradient(cr, orc):
+    """Vertex gradients with hexagons and pentagons (6 slots, pentagons padded)
+    as the clip side: the intersection areas do not depend on which side
+    clips, so dL/dV of both grids equals the quad-clipped result, and central
+    differences of the oracle's loss built with the hexagons clipping."""
+    _case("hexagon_clip_grad")
+    (v, off, idx), hcells = _hexes(cr, 3, seed=7)
+    D = cr.lonlat_points(48, 24, lon0=0.5)
+    dcells = cr.points_to_cells(D)
+    rng = np.random.default_rng(11)
+    s, t = rng.random(off.size - 1), rng.random(48 * 24)
+    src, dst = cr.Grid.from_polygons(v, off, idx), cr.Grid.from_points(D)
+    st = torch.tensor(s, device=DEV)
+    out = {}
+    for side in ("dst", "src"):
+        R = cr.Regridder(dst, src, radius=1.0, clip_side=side)
+        assert R.stats()["clip_with_dst"] == (side == "dst")
+        d = cr.regrid(R, st)
+        gy = 2.0 * (d - torch.tensor(t, device=DEV))
+        _, g_dst, g_src = cr.regrid_backward(R, dst, src, st, d, gy, field_grad=False)
+        out[side] = (g_dst.cpu().numpy(), g_src.cpu().numpy())
+    for a, b in zip(out["dst"], out["src"]):
+        np.testing.assert_allclose(b, a, rtol=0, atol=1e-9 * np.abs(a).max())
+    g_src = out["src"][1]
+
+    def loss(vv):
+        ref = orc.build(dcells, cr.polygons_to_cells(vv, off, idx)[0], clip_is_dst=False)
+        return float(np.sum((ref.apply(s) - t) ** 2))
+
+    h, worst = 1e-6, 0.0
+    for p in rng.choice(v.shape[0], 8, replace=False):
+        for k in range(3):
+            vp, vm = v.copy(), v.copy()
+            vp[p, k] += h
+            vm[p, k] -= h
+            vp[p] /= np.linalg.norm(vp[p])
+            vm[p] /= np.linalg.norm(vm[p])
+            fd = (loss(vp) - loss(vm)) / (2 * h)
+            scale = max(abs(fd), np.abs(g_src[p]).max(), 1e-3)
+            worst = max(worst, abs(g_src[p, k] - fd) / scale)
+    assert worst < 1e-5, worst
