@@ -445,46 +445,72 @@ __global__ void __launch_bounds__(64) k_area_grad_private(const double *__restri
 }
 
 // The same for tables of NV-vertex cells whose slot -> vertex map is the
-// identity (cells own their vertices and none was re-oriented): a lane keeps
-// its cell's NV slot gradients in registers (the fan's slots are compile-time
-// indices), and one LDS buffer stages the block's vertices in and the
-// gradients out, both with coalesced accesses.
+// identity (cells own their vertices and none was re-oriented): two lanes per
+// cell, each taking half of its fan (triangles (v0, v_i, v_{i+1}) below and
+// from the middle vertex v_H on) with its slots' gradients in registers (the
+// fan's slots are compile-time indices); the two shares of v0 and v_H meet by
+// a lane shuffle.  One LDS buffer stages the block's 64 cells' vertices in
+// and their gradients out, both with coalesced accesses (two lanes a cell:
+// twice the waves one lane a cell had for the same LDS).
 template <int NV>
-__global__ void __launch_bounds__(64) k_area_grad_table(const double *__restrict__ cells, int64_t ncells,
-                                                        double kappa, const double *__restrict__ a_to, int64_t nf,
-                                                        const double *__restrict__ y, int64_t yc, int64_t yf,
-                                                        const double *__restrict__ gy, int64_t gc, int64_t gf,
-                                                        double *__restrict__ g) {
+__global__ void __launch_bounds__(128) k_area_grad_table(const double *__restrict__ cells, int64_t ncells,
+                                                         double kappa, const double *__restrict__ a_to, int64_t nf,
+                                                         const double *__restrict__ y, int64_t yc, int64_t yf,
+                                                         const double *__restrict__ gy, int64_t gc, int64_t gf,
+                                                         double *__restrict__ g) {
+    constexpr int H = NV / 2;
+    static_assert(NV >= 4 && NV % 2 == 0, "even rings of at least 4 vertices");
     __shared__ double stage[64 * NV * 3];
     const int64_t c0 = (int64_t)blockIdx.x * 64;
     const int ncell = (int)min((int64_t)64, ncells - c0);
     const int n = ncell * NV * 3;
     const double *src = cells + 3 * (int64_t)NV * c0;
     double *gb = g + 3 * (int64_t)NV * c0;
-    for (int i = threadIdx.x; i < n; i += 64) stage[i] = src[i];
+    for (int i = threadIdx.x; i < n; i += 128) stage[i] = src[i];
     __syncthreads();
-    const int t = threadIdx.x;
+    const int t = threadIdx.x >> 1;  // the cell; lanes 2t and 2t + 1 share it (one wave)
+    const bool hi = (threadIdx.x & 1) != 0;
     const int64_t c = c0 + t;
-    d3 gr[NV];
+    d3 ga = mk3(0.0, 0.0, 0.0);
+    d3 gr[H];  // the lane's slots: 1 .. H (low lane) or H .. NV-1 (high lane)
 #pragma unroll
-    for (int k = 0; k < NV; ++k) gr[k] = mk3(0.0, 0.0, 0.0);
+    for (int k = 0; k < H; ++k) gr[k] = mk3(0.0, 0.0, 0.0);
     if (t < ncell && a_to[c] != 0.0) {  // zero-area (ghost) cells have no area term
         double sum = 0.0;
         for (int64_t f = 0; f < nf; ++f) sum += gy[c * gc + f * gf] * y[c * yc + f * yf];
         const double coef = -kappa * sum / a_to[c];
-        if (coef != 0.0) ring_area_backward_n<NV>(stage + 3 * NV * t, coef, gr);
+        if (coef != 0.0) {
+            // the low lane: triangles 1 .. H-1 (slots 1 .. H), the high lane:
+            // triangles H .. NV-2 (slots H .. NV-1)
+            if (!hi) ring_area_backward_part<1, H>(stage + 3 * NV * t, coef, ga, gr);
+            else ring_area_backward_part<H, NV - 1>(stage + 3 * NV * t, coef, ga, gr);
+        }
     }
+    // the shared slots: v0 from both lanes, v_H (the low lane's last, the high lane's first)
+    const d3 ga_o = mk3(__shfl_xor(ga.x, 1), __shfl_xor(ga.y, 1), __shfl_xor(ga.z, 1));
+    const d3 mid = hi ? gr[0] : gr[H - 1];
+    const d3 mid_o = mk3(__shfl_xor(mid.x, 1), __shfl_xor(mid.y, 1), __shfl_xor(mid.z, 1));
     __syncthreads();
     if (t < ncell) {
+        double *o = stage + 3 * NV * t;
+        auto put = [&](int k, const d3 &v) {
+            o[3 * k] = v.x;
+            o[3 * k + 1] = v.y;
+            o[3 * k + 2] = v.z;
+        };
+        if (!hi) {
+            const double *a = o;  // v0, still staged: the low lane writes it only now
+            put(0, tangential(ga + ga_o, load3(a)));
 #pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            stage[3 * (NV * t + k)] = gr[k].x;
-            stage[3 * (NV * t + k) + 1] = gr[k].y;
-            stage[3 * (NV * t + k) + 2] = gr[k].z;
+            for (int k = 1; k < H; ++k) put(k, gr[k - 1]);
+            put(H, gr[H - 1] + mid_o);
+        } else {
+#pragma unroll
+            for (int k = H + 1; k < NV; ++k) put(k, gr[k - H]);
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += 64) gb[i] += stage[i];
+    for (int i = threadIdx.x; i < n; i += 128) gb[i] += stage[i];
 }
 
 __global__ void k_div_rows(const double *__restrict__ gy, int64_t gc, int64_t gf, int64_t n, int64_t nf,
@@ -613,13 +639,13 @@ static void backward(const cr_regridder *R, const cr_grid *dst, const cr_grid *s
     if (normalize && g_to) {
         const bool table = gto->private_vertices && gto->flipped == 0;  // slot -> vertex is the identity
         if (table && gto->nv == 4)
-            hipLaunchKernelGGL(k_area_grad_table<4>, dim3(grid_for(n_to, 64)), dim3(64), 0, s, gto->cells.p, n_to,
+            hipLaunchKernelGGL(k_area_grad_table<4>, dim3(grid_for(n_to, 64)), dim3(128), 0, s, gto->cells.p, n_to,
                                kappa, a_to, nfields, y, yc, yf, gy, gc, gf, g_to);
         else if (table && gto->nv == 8)
-            hipLaunchKernelGGL(k_area_grad_table<8>, dim3(grid_for(n_to, 64)), dim3(64), 0, s, gto->cells.p, n_to,
+            hipLaunchKernelGGL(k_area_grad_table<8>, dim3(grid_for(n_to, 64)), dim3(128), 0, s, gto->cells.p, n_to,
                                kappa, a_to, nfields, y, yc, yf, gy, gc, gf, g_to);
         else if (table && gto->nv == 16)
-            hipLaunchKernelGGL(k_area_grad_table<16>, dim3(grid_for(n_to, 64)), dim3(64), 0, s, gto->cells.p,
+            hipLaunchKernelGGL(k_area_grad_table<16>, dim3(grid_for(n_to, 64)), dim3(128), 0, s, gto->cells.p,
                                n_to, kappa, a_to, nfields, y, yc, yf, gy, gc, gf, g_to);
         else if (gto->private_vertices && gto->nv <= kStagedNv)
             hipLaunchKernelGGL(k_area_grad_private, dim3(grid_for(n_to, 64)), dim3(64),

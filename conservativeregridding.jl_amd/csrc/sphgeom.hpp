@@ -902,26 +902,30 @@ CR_HD void ring_area_backward(const double *xyz, int n, double coef, Emit &&emit
     emit(0, tangential(ga, a));
 }
 
-// ring_area_backward for rings of NV vertices, the slot gradients added to
-// g[NV] (tangential; unrolled, so the slots are compile-time indices).
-template <int NV>
-CR_HD void ring_area_backward_n(const double *xyz, double coef, d3 (&g)[NV]) {
+// Part of ring_area_backward's fan, unrolled (the slots are compile-time
+// indices): the triangles (v0, v_i, v_{i+1}) for I0 <= i < I1.  ga (not made
+// tangential) gets their derivative w.r.t. v0, g[j] (made
+// tangential) w.r.t. v_{I0+j}, 0 <= j <= I1 - I0.  Two parts that meet at a
+// vertex each hold a share of its derivative; the projection is linear, so
+// the shares add.
+template <int I0, int I1>
+CR_HD void ring_area_backward_part(const double *xyz, double coef, d3 &ga, d3 (&g)[I1 - I0 + 1]) {
     const d3 a = load3(xyz);
-    d3 ga = mk3(0, 0, 0), pend = mk3(0, 0, 0);
-    d3 b = load3(xyz + 3);
+    d3 pend = mk3(0, 0, 0);
+    d3 b = load3(xyz + 3 * I0);
 #pragma unroll
-    for (int i = 1; i + 1 < NV; ++i) {
+    for (int i = I0; i < I1; ++i) {
         const d3 c = load3(xyz + 3 * (i + 1));
         const double T = dot(a, cross(b, c));
         const double D = 1.0 + dot(a, b) + dot(b, c) + dot(c, a);
         const double f = coef * 2.0 / (T * T + D * D);
         ga = ga + f * (D * cross(b, c) - T * (b + c));
-        g[i] = g[i] + tangential(pend + f * (D * cross(c, a) - T * (a + c)), b);
+        g[i - I0] = g[i - I0] + tangential(pend + f * (D * cross(c, a) - T * (a + c)), b);
         pend = f * (D * cross(a, b) - T * (a + b));
         b = c;
     }
-    g[NV - 1] = g[NV - 1] + tangential(pend, b);
-    g[0] = g[0] + tangential(ga, a);
+    g[I1 - I0] = g[I1 - I0] + tangential(pend, b);
 }
+
 
 }  // namespace cr
