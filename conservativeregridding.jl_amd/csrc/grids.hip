@@ -718,7 +718,7 @@ cr_status cr_grid_from_points(cr_grid **grid, const double *pts, int64_t nblocks
         check_device();
         hipStream_t s = as_stream(stream);
         int64_t npts = nblocks * (ni + 1) * (nj + 1);
-        DevBuf<double> dpts((size_t)npts * 3);
+        DevBuf<double> dpts((size_t)npts * 3, s);  // pool memory, freed in order on s
         CR_HIP(hipMemcpyAsync(dpts.p, pts, dpts.bytes(), hipMemcpyHostToDevice, s));
         {
             DevBuf<unsigned long long> res(2, s);
