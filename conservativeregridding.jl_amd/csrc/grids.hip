@@ -132,7 +132,7 @@ void launch_planar_areas(const cr_grid *g, double *area, hipStream_t s) {
 }
 
 void orient_grid(cr_grid *g, hipStream_t s) {
-    DevBuf<int> cnt(1);
+    DevBuf<int> cnt(1, s);
     CR_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int), s));
     hipLaunchKernelGGL(k_orient_cells, dim3(grid_for(g->ncells, 256)), dim3(256), 0, s, g->cells.p,
                        g->vidx.p, g->ncells, g->nv, cnt.p);
@@ -762,7 +762,7 @@ cr_status cr_grid_tripolar(cr_grid **grid, const double *pts, int64_t nx, int64_
         check_device();
         hipStream_t s = as_stream(stream);
         const int64_t npts = (nx + 1) * (ny + 1);
-        DevBuf<double> dpts((size_t)npts * 3);
+        DevBuf<double> dpts((size_t)npts * 3, s);
         CR_HIP(hipMemcpyAsync(dpts.p, pts, dpts.bytes(), hipMemcpyHostToDevice, s));
         auto *g = new cr_grid();
         try {
