@@ -6,7 +6,10 @@
 #include <functional>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <thread>
+
+#include <pthread.h>
 #include <vector>
 
 #include "gridgen.hpp"
@@ -146,9 +149,9 @@ void orient_grid(cr_grid *g, hipStream_t s) {
 // Host worker threads of the input checks, started once and asleep between
 // calls (starting 16 threads per call cost 0.3 ms of the 0.35 ms unit check of
 // a 1 M-point matrix).  A call's range is cut into chunks that the workers and
-// the calling thread take from one counter, so a call completes even where no
-// worker runs (a forked child); a call that finds the workers busy with
-// another thread's call runs its range alone.
+// the calling thread take from one counter; a call that finds the workers
+// busy with another thread's call runs its range alone.  A forked child has
+// no workers: its pool is reset to run every call on the calling thread.
 class HostPool {
   public:
     static HostPool &get() {
@@ -189,6 +192,17 @@ class HostPool {
         const unsigned hw = std::thread::hardware_concurrency();
         nworkers_ = std::max(1u, std::min(16u, hw)) - 1;
         for (unsigned t = 0; t < nworkers_; ++t) std::thread([this] { loop(); }).detach();
+        pthread_atfork(nullptr, nullptr, [] { HostPool::get().reset_in_child(); });
+    }
+    // the child of a fork: the workers did not follow, and a lock held at the
+    // fork stays held; fresh locks, no workers
+    void reset_in_child() {
+        new (&call_mu_) std::mutex();
+        new (&mu_) std::mutex();
+        new (&cv_) std::condition_variable();
+        new (&done_cv_) std::condition_variable();
+        new (&job_) std::shared_ptr<Job>();
+        nworkers_ = 0;
     }
     void work(Job &job) {
         int64_t finished = 0;
